@@ -1,0 +1,16 @@
+"""shocovox_amd: MI355X-native ray traversal of shocovox sparse voxel octrees.
+
+The traversal hot path (Octree::get_by_ray) runs as hand-written HIP
+kernels for gfx950 inside libsvo.so, behind the C ABI of include/svo.h.
+"""
+from .octree import (  # noqa: F401
+    COMPLEX, EMPTY, EMPTY_MARKER, INFORMATIVE, VISUAL, DeviceTree, FlatTree, Octree,
+    OctreeEntry, OctreeError, Viewport, albedo, albedo_rgba, alloc_hits, collect_kernel_ms,
+    set_timing, voxel_data,
+)
+
+__all__ = [
+    "COMPLEX", "EMPTY", "EMPTY_MARKER", "INFORMATIVE", "VISUAL", "DeviceTree", "FlatTree",
+    "Octree", "OctreeEntry", "OctreeError", "Viewport", "albedo", "albedo_rgba", "alloc_hits",
+    "collect_kernel_ms", "set_timing", "voxel_data",
+]

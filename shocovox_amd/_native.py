@@ -1,0 +1,128 @@
+"""ctypes binding of libsvo.so (declarations in include/svo.h).
+
+The library is built in-tree by shocovox_amd.build; importing this module
+never falls back to any other implementation: a missing or stale library
+raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libsvo.so"
+
+SVO_OK = 0
+STATUS_NAMES = {
+    0: "OK", 1: "InvalidSize", 2: "InvalidBrickDimension", 3: "InvalidStructure",
+    4: "InvalidPosition", 5: "InvalidArgument", 6: "OutOfDeviceMemory", 7: "HipError",
+    8: "NoDevice",
+}
+
+
+class SvoError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Entry(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("albedo", C.c_uint32), ("data", C.c_uint32)]
+
+
+class ModelSizes(C.Structure):
+    _fields_ = [("octree_size", C.c_uint32), ("brick_dim", C.c_uint32), ("pool_len", C.c_uint32),
+                ("parted_count", C.c_uint32), ("color_count", C.c_uint32),
+                ("data_count", C.c_uint32)]
+
+
+class ModelDump(C.Structure):
+    _fields_ = [("reserved", C.c_void_p), ("content_kind", C.c_void_p),
+                ("internal_bits", C.c_void_p), ("children_kind", C.c_void_p),
+                ("children", C.c_void_p), ("children_bitmap", C.c_void_p),
+                ("brick_kind", C.c_void_p), ("brick_value", C.c_void_p), ("voxels", C.c_void_p),
+                ("colors", C.c_void_p), ("data", C.c_void_p)]
+
+
+class TreeDesc(C.Structure):
+    _fields_ = [("octree_size", C.c_uint32), ("brick_dim", C.c_uint32),
+                ("node_count", C.c_uint32), ("brick_count", C.c_uint32),
+                ("color_count", C.c_uint32), ("top_level_nodes", C.c_uint32),
+                ("nodes", C.c_void_p), ("brick_pix", C.c_void_p), ("brick_occ", C.c_void_p),
+                ("colors", C.c_void_p)]
+
+
+class Hits(C.Structure):
+    _fields_ = [("hit", C.c_void_p), ("pix", C.c_void_p), ("albedo", C.c_void_p),
+                ("voxel", C.c_void_p), ("distance", C.c_void_p), ("impact", C.c_void_p),
+                ("normal", C.c_void_p), ("flags", C.c_void_p)]
+
+
+class Viewport(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("direction", C.c_float * 3),
+                ("frustum", C.c_float * 3), ("fov", C.c_float)]
+
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_SIGS = {
+    "svo_last_error": (C.c_char_p, []),
+    "svo_version": (C.c_char_p, []),
+    "svo_octree_new": (C.c_int, [_U32, _U32, C.POINTER(_P)]),
+    "svo_octree_free": (None, [_P]),
+    "svo_octree_get_size": (_U32, [_P]),
+    "svo_octree_get_brick_dim": (_U32, [_P]),
+    "svo_octree_set_auto_simplify": (C.c_int, [_P, C.c_int]),
+    "svo_octree_insert": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Entry)]),
+    "svo_octree_insert_at_lod": (C.c_int, [_P, _U32, _U32, _U32, _U32, C.POINTER(Entry)]),
+    "svo_octree_update": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Entry)]),
+    "svo_octree_clear": (C.c_int, [_P, _U32, _U32, _U32]),
+    "svo_octree_clear_at_lod": (C.c_int, [_P, _U32, _U32, _U32, _U32]),
+    "svo_octree_get": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Entry)]),
+    "svo_octree_insert_many": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "svo_octree_clear_many": (C.c_int, [_P, _P, C.c_size_t]),
+    "svo_octree_get_many": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "svo_octree_export": (C.c_int, [_P, C.POINTER(ModelSizes), C.POINTER(ModelDump)]),
+    "svo_scene_build": (C.c_int, [_U32, _U32, _U32, C.c_uint64, C.POINTER(_P)]),
+    "svo_octree_flatten": (C.c_int, [_P, C.POINTER(_P)]),
+    "svo_flat_desc": (C.c_int, [_P, C.POINTER(TreeDesc)]),
+    "svo_flat_free": (None, [_P]),
+    "svo_flat_get": (C.c_int, [_P, _P, _U32, _U32, _U32, C.POINTER(Entry)]),
+    "svo_tree_create": (C.c_int, [C.POINTER(TreeDesc), C.POINTER(_P)]),
+    "svo_tree_from_octree": (C.c_int, [_P, C.POINTER(_P)]),
+    "svo_tree_update": (C.c_int, [_P, _P, _P]),
+    "svo_tree_destroy": (None, [_P]),
+    "svo_tree_device_bytes": (C.c_uint64, [_P]),
+    "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),
+    "svo_render": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.POINTER(Hits), _P]),
+    "svo_ray_occupancy_lut": (C.c_int, [_P]),
+    "svo_set_timing": (C.c_int, [C.c_int]),
+    "svo_collect_kernel_ms": (_U32, [C.POINTER(C.c_float), _U32]),
+}
+
+_lib = None
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """Loads libsvo.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m shocovox_amd.build`")
+    handle = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = handle
+    return handle
+
+
+def check(status: int) -> None:
+    if status != SVO_OK:
+        msg = lib().svo_last_error()
+        raise SvoError(status, msg.decode() if msg else "")

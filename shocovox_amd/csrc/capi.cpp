@@ -1,0 +1,360 @@
+// extern "C" boundary of libsvo (include/svo.h).
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/svo.h"
+#include "device.hpp"
+#include "flatten.hpp"
+#include "octree.hpp"
+
+namespace svo {
+Status build_scene(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t seed, Octree* out);
+}
+
+struct svo_octree {
+  svo::Octree tree;
+};
+
+struct svo_flat {
+  svo::FlatTree flat;
+};
+
+struct svo_tree {
+  svo::DeviceTree* dev = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int status, const std::string& msg) {
+  g_error = msg;
+  return status;
+}
+
+int from_status(svo::Status s) {
+  switch (s) {
+    case svo::Status::Ok:
+      return SVO_OK;
+    case svo::Status::InvalidSize:
+      return fail(SVO_INVALID_SIZE, "invalid octree size");
+    case svo::Status::InvalidBrickDimension:
+      return fail(SVO_INVALID_BRICK_DIMENSION, "invalid brick dimension");
+    case svo::Status::InvalidStructure:
+      return fail(SVO_INVALID_STRUCTURE, "invalid structure");
+    case svo::Status::InvalidPosition:
+      return fail(SVO_INVALID_POSITION, "position outside the octree");
+  }
+  return fail(SVO_INVALID_ARGUMENT, "unknown status");
+}
+
+svo::Entry to_entry(const svo_entry* e) {
+  svo::Entry r;
+  if (!e) return r;
+  r.kind = static_cast<svo::EntryKind>(e->kind & 3u);
+  r.albedo = e->albedo;
+  r.data = e->data;
+  return r;
+}
+
+void from_entry(const svo::Entry& e, svo_entry* out) {
+  out->kind = static_cast<uint32_t>(e.kind);
+  out->albedo = e.albedo;
+  out->data = e.data;
+}
+
+void fill_desc(const svo::FlatTree& f, svo_tree_desc* d) {
+  d->octree_size = f.octree_size;
+  d->brick_dim = f.brick_dim;
+  d->node_count = static_cast<uint32_t>(f.nodes.size());
+  d->brick_count = f.brick_count();
+  d->color_count = static_cast<uint32_t>(f.colors.size());
+  d->top_level_nodes = 0;
+  for (uint32_t e : f.level_end) {
+    if (e > 2048) break;
+    d->top_level_nodes = e;
+  }
+  d->nodes = reinterpret_cast<const svo_node_rec*>(f.nodes.data());
+  d->brick_pix = f.brick_pix.data();
+  d->brick_occ = f.brick_occ.data();
+  d->colors = f.colors.data();
+}
+
+}  // namespace
+
+static_assert(sizeof(svo_node_rec) == sizeof(svo::NodeRec), "node record layout");
+
+extern "C" {
+
+const char* svo_last_error(void) { return g_error.c_str(); }
+const char* svo_version(void) { return "svo-mi355x 0.1.0"; }
+
+int svo_octree_new(uint32_t size, uint32_t brick_dim, svo_octree** out) {
+  if (!out) return fail(SVO_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  svo::Status st;
+  svo::Octree t = svo::Octree::create(size, brick_dim, &st);
+  if (st != svo::Status::Ok) return from_status(st);
+  svo_octree* h = new (std::nothrow) svo_octree{std::move(t)};
+  if (!h) return fail(SVO_INVALID_ARGUMENT, "allocation failed");
+  *out = h;
+  return SVO_OK;
+}
+
+void svo_octree_free(svo_octree* tree) { delete tree; }
+
+uint32_t svo_octree_get_size(const svo_octree* tree) { return tree ? tree->tree.size() : 0; }
+uint32_t svo_octree_get_brick_dim(const svo_octree* tree) {
+  return tree ? tree->tree.brick_dim() : 0;
+}
+
+int svo_octree_set_auto_simplify(svo_octree* tree, int enabled) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  tree->tree.auto_simplify = enabled != 0;
+  return SVO_OK;
+}
+
+int svo_octree_insert(svo_octree* tree, uint32_t x, uint32_t y, uint32_t z, const svo_entry* e) {
+  if (!tree || !e) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  return from_status(tree->tree.insert(x, y, z, to_entry(e)));
+}
+
+int svo_octree_insert_at_lod(svo_octree* tree, uint32_t x, uint32_t y, uint32_t z,
+                             uint32_t insert_size, const svo_entry* e) {
+  if (!tree || !e) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  return from_status(tree->tree.insert_at_lod(x, y, z, insert_size, to_entry(e)));
+}
+
+int svo_octree_update(svo_octree* tree, uint32_t x, uint32_t y, uint32_t z, const svo_entry* e) {
+  if (!tree || !e) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  return from_status(tree->tree.update(x, y, z, to_entry(e)));
+}
+
+int svo_octree_clear(svo_octree* tree, uint32_t x, uint32_t y, uint32_t z) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  return from_status(tree->tree.clear(x, y, z));
+}
+
+int svo_octree_clear_at_lod(svo_octree* tree, uint32_t x, uint32_t y, uint32_t z,
+                            uint32_t clear_size) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  return from_status(tree->tree.clear_at_lod(x, y, z, clear_size));
+}
+
+int svo_octree_get(const svo_octree* tree, uint32_t x, uint32_t y, uint32_t z, svo_entry* out) {
+  if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  from_entry(tree->tree.get(x, y, z), out);
+  return SVO_OK;
+}
+
+int svo_octree_insert_many(svo_octree* tree, const uint32_t* xyz, const svo_entry* entries,
+                           size_t n) {
+  if (!tree || (n && (!xyz || !entries))) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  int first = SVO_OK;
+  for (size_t i = 0; i < n; ++i) {
+    const int st = from_status(
+        tree->tree.insert(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], to_entry(&entries[i])));
+    if (st != SVO_OK && first == SVO_OK) first = st;
+  }
+  return first;
+}
+
+int svo_octree_clear_many(svo_octree* tree, const uint32_t* xyz, size_t n) {
+  if (!tree || (n && !xyz)) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  int first = SVO_OK;
+  for (size_t i = 0; i < n; ++i) {
+    const int st = from_status(tree->tree.clear(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    if (st != SVO_OK && first == SVO_OK) first = st;
+  }
+  return first;
+}
+
+int svo_octree_get_many(const svo_octree* tree, const uint32_t* xyz, size_t n, svo_entry* out) {
+  if (!tree || (n && (!xyz || !out))) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  for (size_t i = 0; i < n; ++i)
+    from_entry(tree->tree.get(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]), &out[i]);
+  return SVO_OK;
+}
+
+int svo_octree_export(const svo_octree* tree, svo_model_sizes* sizes, svo_model_dump* dump) {
+  if (!tree || !sizes) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const svo::Octree& t = tree->tree;
+  const svo::NodePool& pool = t.nodes();
+  const auto& kids = t.node_children();
+  const uint32_t dim = t.brick_dim();
+  const size_t n3 = static_cast<size_t>(dim) * dim * dim;
+  uint32_t parted = 0;
+  for (uint32_t k = 0; k < pool.len(); ++k) {
+    const svo::NodeContent& n = pool.get(k);
+    const int nb = n.kind == svo::NodeKind::Leaf ? 8 : (n.kind == svo::NodeKind::UniformLeaf ? 1 : 0);
+    for (int o = 0; o < nb; ++o)
+      if (n.bricks[o].kind == svo::BrickKind::Parted) ++parted;
+  }
+  sizes->octree_size = t.size();
+  sizes->brick_dim = dim;
+  sizes->pool_len = pool.len();
+  sizes->parted_count = parted;
+  sizes->color_count = static_cast<uint32_t>(t.color_palette().size());
+  sizes->data_count = static_cast<uint32_t>(t.data_palette().size());
+  if (!dump) return SVO_OK;
+  uint32_t next_parted = 0;
+  for (uint32_t k = 0; k < pool.len(); ++k) {
+    const svo::NodeContent& n = pool.get(k);
+    dump->reserved[k] = pool.reserved(k) ? 1 : 0;
+    dump->content_kind[k] = static_cast<uint8_t>(n.kind);
+    dump->internal_bits[k] = n.kind == svo::NodeKind::Internal ? n.bits : 0;
+    const svo::NodeChildren nc = k < kids.size() ? kids[k] : svo::NodeChildren{};
+    dump->children_kind[k] = static_cast<uint8_t>(nc.kind);
+    for (int o = 0; o < 8; ++o) dump->children[8 * k + o] = nc.c[o];
+    dump->children_bitmap[k] = nc.bitmap;
+    const int nb = n.kind == svo::NodeKind::Leaf ? 8 : (n.kind == svo::NodeKind::UniformLeaf ? 1 : 0);
+    for (int o = 0; o < 8; ++o) {
+      dump->brick_kind[8 * k + o] = 0;
+      dump->brick_value[8 * k + o] = 0;
+      if (o >= nb) continue;
+      const svo::Brick& b = n.bricks[o];
+      dump->brick_kind[8 * k + o] = static_cast<uint8_t>(b.kind);
+      if (b.kind == svo::BrickKind::Solid) {
+        dump->brick_value[8 * k + o] = b.solid;
+      } else if (b.kind == svo::BrickKind::Parted) {
+        dump->brick_value[8 * k + o] = next_parted;
+        std::memcpy(dump->voxels + static_cast<size_t>(next_parted) * n3, b.parted.data(),
+                    n3 * sizeof(uint32_t));
+        ++next_parted;
+      }
+    }
+  }
+  if (sizes->color_count)
+    std::memcpy(dump->colors, t.color_palette().data(), sizes->color_count * sizeof(uint32_t));
+  if (sizes->data_count)
+    std::memcpy(dump->data, t.data_palette().data(), sizes->data_count * sizeof(uint32_t));
+  return SVO_OK;
+}
+
+int svo_scene_build(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t seed,
+                    svo_octree** out) {
+  if (!out) return fail(SVO_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  svo_octree* h = new (std::nothrow) svo_octree{};
+  if (!h) return fail(SVO_INVALID_ARGUMENT, "allocation failed");
+  const svo::Status st = svo::build_scene(kind, size, brick_dim, seed, &h->tree);
+  if (st != svo::Status::Ok) {
+    delete h;
+    return from_status(st);
+  }
+  *out = h;
+  return SVO_OK;
+}
+
+int svo_octree_flatten(const svo_octree* tree, svo_flat** out) {
+  if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  svo_flat* f = new (std::nothrow) svo_flat{};
+  if (!f) return fail(SVO_INVALID_ARGUMENT, "allocation failed");
+  f->flat = svo::flatten(tree->tree);
+  *out = f;
+  return SVO_OK;
+}
+
+int svo_flat_desc(const svo_flat* flat, svo_tree_desc* out) {
+  if (!flat || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  fill_desc(flat->flat, out);
+  return SVO_OK;
+}
+
+void svo_flat_free(svo_flat* flat) { delete flat; }
+
+int svo_flat_get(const svo_flat* flat, const svo_octree* tree, uint32_t x, uint32_t y, uint32_t z,
+                 svo_entry* out) {
+  if (!flat || !tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const uint32_t pix = svo::flat_get(flat->flat, tree->tree, x, y, z);
+  from_entry(tree->tree.pix_get_ref(pix), out);
+  return SVO_OK;
+}
+
+int svo_tree_create(const svo_tree_desc* desc, svo_tree** out) {
+  if (!desc || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  svo::DeviceTree* dev = nullptr;
+  std::string err;
+  const int st = svo::device_tree_create(*desc, &dev, &err);
+  if (st != SVO_OK) return fail(st, err);
+  *out = new svo_tree{dev};
+  return SVO_OK;
+}
+
+int svo_tree_from_octree(const svo_octree* tree, svo_tree** out) {
+  if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const svo::FlatTree f = svo::flatten(tree->tree);
+  svo_tree_desc d;
+  fill_desc(f, &d);
+  return svo_tree_create(&d, out);
+}
+
+int svo_tree_update(svo_tree* dtree, const svo_octree* tree, void* stream) {
+  if (!dtree || !tree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const svo::FlatTree f = svo::flatten(tree->tree);
+  svo_tree_desc d;
+  fill_desc(f, &d);
+  std::string err;
+  const int st = svo::device_tree_upload(dtree->dev, d, stream, &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
+void svo_tree_destroy(svo_tree* dtree) {
+  if (!dtree) return;
+  svo::device_tree_destroy(dtree->dev);
+  delete dtree;
+}
+
+uint64_t svo_tree_device_bytes(const svo_tree* dtree) {
+  return dtree && dtree->dev ? dtree->dev->bytes : 0;
+}
+
+int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hits* out,
+             void* stream) {
+  if (!dtree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::string err;
+  const int st = svo::device_cast(*dtree->dev, rays, n, *out, stream, &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
+int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
+               const svo_hits* out, void* stream) {
+  if (!dtree || !vp || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  using svo::V3f;
+  const V3f origin{vp->origin[0], vp->origin[1], vp->origin[2]};
+  const V3f dir{vp->direction[0], vp->direction[1], vp->direction[2]};
+  const V3f up{0.0f, 1.0f, 0.0f};
+  svo::Camera cam;
+  cam.origin = origin;
+  cam.up = up;
+  cam.right = svo::normalized(svo::cross(up, dir));
+  cam.pixel_w = vp->frustum[0] / static_cast<float>(w);
+  cam.pixel_h = vp->frustum[1] / static_cast<float>(h);
+  cam.bottom_left = ((origin + dir * vp->fov) - up * (vp->frustum[1] / 2.0f)) -
+                    cam.right * (vp->frustum[0] / 2.0f);
+  std::string err;
+  const int st = svo::device_render(*dtree->dev, cam, w, h, *out, stream, &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
+int svo_ray_occupancy_lut(uint64_t* out512) {
+  if (!out512) return fail(SVO_INVALID_ARGUMENT, "out is NULL");
+  svo::generate_ray_occupancy_lut(out512);
+  return SVO_OK;
+}
+
+int svo_set_timing(int enabled) {
+  svo::timing_enable(enabled != 0);
+  return SVO_OK;
+}
+
+uint32_t svo_collect_kernel_ms(float* out_ms, uint32_t cap) {
+  return svo::timing_collect(out_ms, cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,342 @@
+// gfx950 kernels of the traversal path and device residency of a tree.
+//
+// cast_kernel: one ray per lane from a caller-provided ray array.
+// render_kernel: primary rays generated on the device; each 64-lane wave
+//   owns an 8x8 pixel tile so neighbouring lanes follow nearby paths through
+//   the tree and share node and brick cache lines.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "device.hpp"
+#include "flatten.hpp"
+
+namespace svo {
+
+namespace {
+
+struct HitPtrs {
+  uint32_t* hit;
+  uint32_t* pix;
+  uint32_t* albedo;
+  uint32_t* voxel;
+  float* distance;
+  float* impact;
+  float* normal;
+  uint32_t* flags;
+};
+
+HitPtrs to_ptrs(const svo_hits& h) {
+  return HitPtrs{h.hit, h.pix, h.albedo, h.voxel, h.distance, h.impact, h.normal, h.flags};
+}
+
+__device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint32_t i,
+                                    const CastResult& r, V3f org) {
+  if (o.hit) o.hit[i] = r.hit;
+  if (o.pix) o.pix[i] = r.hit ? r.pix : EMPTY_MARKER;
+  if (o.albedo) {
+    const uint32_t c = pix_color_index(r.pix);
+    o.albedo[i] = (r.hit && c < NONE16 && c < t.color_count) ? t.colors[c] : 0u;
+  }
+  if (o.voxel) {
+    if (r.hit) {
+      o.voxel[3 * i + 0] = hit_voxel_axis(r.impact.x, r.cell.min.x, r.cell.size);
+      o.voxel[3 * i + 1] = hit_voxel_axis(r.impact.y, r.cell.min.y, r.cell.size);
+      o.voxel[3 * i + 2] = hit_voxel_axis(r.impact.z, r.cell.min.z, r.cell.size);
+    } else {
+      o.voxel[3 * i + 0] = EMPTY_MARKER;
+      o.voxel[3 * i + 1] = EMPTY_MARKER;
+      o.voxel[3 * i + 2] = EMPTY_MARKER;
+    }
+  }
+  if (o.distance) o.distance[i] = r.hit ? length(r.impact - org) : __builtin_inff();
+  if (o.impact) {
+    o.impact[3 * i + 0] = r.impact.x;
+    o.impact[3 * i + 1] = r.impact.y;
+    o.impact[3 * i + 2] = r.impact.z;
+  }
+  if (o.normal) {
+    o.normal[3 * i + 0] = r.normal.x;
+    o.normal[3 * i + 1] = r.normal.y;
+    o.normal[3 * i + 2] = r.normal.z;
+  }
+  if (o.flags) o.flags[i] = r.flags;
+}
+
+__global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __restrict__ rays,
+                                                   uint32_t n, HitPtrs out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rays + 6ull * i;
+  const V3f o{r[0], r[1], r[2]};
+  const V3f d{r[3], r[4], r[5]};
+  const CastResult res = cast_ray(t, o, d);
+  store_result(t, out, i, res, o);
+}
+
+__global__ __launch_bounds__(256) void render_kernel(TreeView t, Camera cam, uint32_t w,
+                                                     uint32_t h, HitPtrs out) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t x = (blockIdx.x * 2u + (wave & 1u)) * 8u + (lane & 7u);
+  const uint32_t y = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane >> 3);
+  if (x >= w || y >= h) return;
+  const V3f d = camera_ray_dir(cam, x, y);
+  const CastResult res = cast_ray(t, cam.origin, d);
+  store_result(t, out, y * w + x, res, cam.origin);
+}
+
+// --- timing ---------------------------------------------------------------
+
+struct EventPair {
+  hipEvent_t start, stop;
+};
+bool g_timing = false;
+std::mutex g_timing_mu;
+std::vector<EventPair> g_events;
+
+struct LaunchTimer {
+  EventPair ev{nullptr, nullptr};
+  hipStream_t s;
+  bool on;
+  explicit LaunchTimer(hipStream_t st) : s(st), on(g_timing) {
+    if (on && hipEventCreate(&ev.start) == hipSuccess && hipEventCreate(&ev.stop) == hipSuccess)
+      (void)hipEventRecord(ev.start, s);
+    else
+      on = false;
+  }
+  void done() {
+    if (!on) return;
+    (void)hipEventRecord(ev.stop, s);
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    g_events.push_back(ev);
+  }
+};
+
+int hip_fail(hipError_t e, const char* what, std::string* err) {
+  *err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? SVO_OUT_OF_DEVICE_MEMORY : SVO_HIP_ERROR;
+}
+
+// Checks that every index the kernels follow stays inside the arrays.
+bool validate_desc(const svo_tree_desc& d, std::string* err) {
+  if (d.node_count == 0 || d.nodes == nullptr) {
+    *err = "tree has no nodes";
+    return false;
+  }
+  if (d.brick_dim == 0 || d.octree_size < 2 * d.brick_dim) {
+    *err = "invalid octree size / brick dimension";
+    return false;
+  }
+  if (d.brick_count && (d.brick_pix == nullptr || d.brick_occ == nullptr)) {
+    *err = "brick arrays missing";
+    return false;
+  }
+  for (uint32_t i = 0; i < d.node_count; ++i) {
+    const svo_node_rec& r = d.nodes[i];
+    const uint32_t kind = r.meta & 3u;
+    if (kind == KIND_INTERNAL) {
+      for (uint32_t o = 0; o < 8; ++o)
+        if (r.child[o] != INVALID_INDEX && r.child[o] >= d.node_count) {
+          *err = "child index out of range";
+          return false;
+        }
+    } else if (kind == KIND_LEAF || kind == KIND_UNIFORM_LEAF) {
+      const uint32_t n = kind == KIND_LEAF ? 8u : 1u;
+      for (uint32_t o = 0; o < n; ++o) {
+        const uint32_t form = brick_form(r.meta, o);
+        if (form == 3u || (form == FORM_PARTED && r.child[o] >= d.brick_count)) {
+          *err = "brick reference out of range";
+          return false;
+        }
+      }
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+void timing_enable(bool on) { g_timing = on; }
+
+uint32_t timing_collect(float* out_ms, uint32_t cap) {
+  std::lock_guard<std::mutex> g(g_timing_mu);
+  uint32_t n = 0;
+  for (const EventPair& e : g_events) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(e.stop) == hipSuccess &&
+        hipEventElapsedTime(&ms, e.start, e.stop) == hipSuccess && n < cap)
+      out_ms[n++] = ms;
+    (void)hipEventDestroy(e.start);
+    (void)hipEventDestroy(e.stop);
+  }
+  g_events.clear();
+  return n;
+}
+
+int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std::string* err) {
+  if (!validate_desc(d, err)) return SVO_INVALID_ARGUMENT;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(t->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
+  const uint64_t n3 = static_cast<uint64_t>(d.brick_dim) * d.brick_dim * d.brick_dim;
+  const uint64_t words = occ_words_per_brick(d.brick_dim);
+  auto grow = [&](void** p, uint64_t* cap, uint64_t need, uint64_t elem) -> int {
+    if (need <= *cap && *p) return SVO_OK;
+    uint64_t nc = need + need / 4 + 1;
+    void* np = nullptr;
+    hipError_t ge = hipMalloc(&np, nc * elem);
+    if (ge != hipSuccess) return hip_fail(ge, "hipMalloc", err);
+    if (*p) {
+      // keep the old buffer alive until queued work on it has finished
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(*p);
+    }
+    *p = np;
+    *cap = nc;
+    return SVO_OK;
+  };
+  uint64_t old_bytes_nodes = t->node_capacity, old_bricks = t->brick_capacity;
+  (void)old_bytes_nodes;
+  (void)old_bricks;
+  int st = grow(reinterpret_cast<void**>(&t->nodes), &t->node_capacity, d.node_count,
+                sizeof(NodeRec));
+  if (st != SVO_OK) return st;
+  uint64_t bcap = t->brick_capacity;
+  st = grow(reinterpret_cast<void**>(&t->brick_pix), &t->brick_capacity,
+            d.brick_count ? d.brick_count : 1, n3 * sizeof(uint32_t));
+  if (st != SVO_OK) return st;
+  if (t->brick_capacity != bcap || t->brick_occ == nullptr) {
+    if (t->brick_occ) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(t->brick_occ);
+      t->brick_occ = nullptr;
+    }
+    hipError_t ge = hipMalloc(reinterpret_cast<void**>(&t->brick_occ),
+                              t->brick_capacity * words * sizeof(uint32_t));
+    if (ge != hipSuccess) return hip_fail(ge, "hipMalloc", err);
+  }
+  st = grow(reinterpret_cast<void**>(&t->colors), &t->color_capacity,
+            d.color_count ? d.color_count : 1, sizeof(uint32_t));
+  if (st != SVO_OK) return st;
+
+  e = hipMemcpyAsync(t->nodes, d.nodes, static_cast<size_t>(d.node_count) * sizeof(NodeRec),
+                     hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(nodes)", err);
+  if (d.brick_count) {
+    e = hipMemcpyAsync(t->brick_pix, d.brick_pix, d.brick_count * n3 * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(bricks)", err);
+    e = hipMemcpyAsync(t->brick_occ, d.brick_occ, d.brick_count * words * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(occupancy)", err);
+  }
+  if (d.color_count) {
+    e = hipMemcpyAsync(t->colors, d.colors, d.color_count * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(colors)", err);
+  }
+  // Host buffers of the description may be released after this call.
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize", err);
+
+  t->node_count = d.node_count;
+  t->brick_count = d.brick_count;
+  t->color_count = d.color_count;
+  t->top_level_nodes = d.top_level_nodes;
+  t->bytes = t->node_capacity * sizeof(NodeRec) + t->brick_capacity * n3 * sizeof(uint32_t) +
+             t->brick_capacity * words * sizeof(uint32_t) + t->color_capacity * sizeof(uint32_t) +
+             64 * 8 * sizeof(uint64_t);
+  t->view.nodes = t->nodes;
+  t->view.brick_pix = t->brick_pix;
+  t->view.brick_occ = t->brick_occ;
+  t->view.colors = t->colors;
+  t->view.ray_lut = t->ray_lut;
+  t->view.size = static_cast<float>(d.octree_size);
+  t->view.dim = d.brick_dim;
+  t->view.occ_words = static_cast<uint32_t>(words);
+  t->view.color_count = d.color_count;
+  return SVO_OK;
+}
+
+int device_tree_create(const svo_tree_desc& d, DeviceTree** out, std::string* err) {
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    *err = "no HIP device available";
+    return SVO_NO_DEVICE;
+  }
+  DeviceTree* t = new DeviceTree();
+  e = hipGetDevice(&t->device);
+  if (e != hipSuccess) {
+    delete t;
+    return hip_fail(e, "hipGetDevice", err);
+  }
+  e = hipMalloc(reinterpret_cast<void**>(&t->ray_lut), 64 * 8 * sizeof(uint64_t));
+  if (e != hipSuccess) {
+    delete t;
+    return hip_fail(e, "hipMalloc(lut)", err);
+  }
+  uint64_t lut[64 * 8];
+  generate_ray_occupancy_lut(lut);
+  e = hipMemcpy(t->ray_lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    device_tree_destroy(t);
+    return hip_fail(e, "hipMemcpy(lut)", err);
+  }
+  const int st = device_tree_upload(t, d, nullptr, err);
+  if (st != SVO_OK) {
+    device_tree_destroy(t);
+    return st;
+  }
+  *out = t;
+  return SVO_OK;
+}
+
+void device_tree_destroy(DeviceTree* t) {
+  if (!t) return;
+  if (t->device >= 0) (void)hipSetDevice(t->device);
+  (void)hipDeviceSynchronize();
+  if (t->nodes) (void)hipFree(t->nodes);
+  if (t->brick_pix) (void)hipFree(t->brick_pix);
+  if (t->brick_occ) (void)hipFree(t->brick_occ);
+  if (t->colors) (void)hipFree(t->colors);
+  if (t->ray_lut) (void)hipFree(t->ray_lut);
+  delete t;
+}
+
+int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hits& out,
+                void* stream, std::string* err) {
+  if (n == 0) return SVO_OK;
+  if (rays == nullptr) {
+    *err = "rays is NULL";
+    return SVO_INVALID_ARGUMENT;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  LaunchTimer timer(s);
+  const uint32_t block = 256;
+  const uint32_t grid = (n + block - 1) / block;
+  hipLaunchKernelGGL(cast_kernel, dim3(grid), dim3(block), 0, s, t.view, rays, n, to_ptrs(out));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "cast_kernel launch", err);
+  timer.done();
+  return SVO_OK;
+}
+
+int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
+                  const svo_hits& out, void* stream, std::string* err) {
+  if (w == 0 || h == 0) return SVO_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  LaunchTimer timer(s);
+  const dim3 grid((w + 15) / 16, (h + 15) / 16);
+  hipLaunchKernelGGL(render_kernel, grid, dim3(256), 0, s, t.view, cam, w, h, to_ptrs(out));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "render_kernel launch", err);
+  timer.done();
+  return SVO_OK;
+}
+
+}  // namespace svo

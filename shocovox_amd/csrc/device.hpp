@@ -1,0 +1,47 @@
+// Device residency and kernel launches of the traversal path.
+#pragma once
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/svo.h"
+#include "layout.hpp"
+#include "traverse.hpp"
+
+namespace svo {
+
+struct DeviceTree {
+  int device = -1;
+  uint32_t node_count = 0;
+  uint32_t brick_count = 0;
+  uint32_t color_count = 0;
+  uint32_t top_level_nodes = 0;
+  uint64_t node_capacity = 0;   // records allocated
+  uint64_t brick_capacity = 0;  // brick slots allocated
+  uint64_t color_capacity = 0;
+  NodeRec* nodes = nullptr;
+  uint32_t* brick_pix = nullptr;
+  uint32_t* brick_occ = nullptr;
+  uint32_t* colors = nullptr;
+  uint64_t* ray_lut = nullptr;
+  uint64_t bytes = 0;
+  TreeView view{};
+};
+
+// Every function returns an svo_status and fills `err` on failure.
+int device_tree_create(const svo_tree_desc& desc, DeviceTree** out, std::string* err);
+// Re-uploads the given description into an existing tree, growing it when
+// needed; ordered on `stream`.
+int device_tree_upload(DeviceTree* t, const svo_tree_desc& desc, void* stream, std::string* err);
+void device_tree_destroy(DeviceTree* t);
+
+int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hits& out,
+                void* stream, std::string* err);
+int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
+                  const svo_hits& out, void* stream, std::string* err);
+
+// Kernel timing with HIP events recorded on the launch stream.
+void timing_enable(bool on);
+uint32_t timing_collect(float* out_ms, uint32_t cap);
+
+}  // namespace svo

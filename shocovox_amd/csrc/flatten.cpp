@@ -1,0 +1,183 @@
+// Host octree -> device layout.  See layout.hpp for the record format.
+#include "flatten.hpp"
+
+#include <cmath>
+#include <unordered_map>
+
+namespace svo {
+
+namespace {
+
+inline bool bound_contains(Cube b, V3f p) {
+  return p.x >= b.min.x && p.x < b.min.x + b.size && p.y >= b.min.y && p.y < b.min.y + b.size &&
+         p.z >= b.min.z && p.z < b.min.z + b.size;
+}
+
+}  // namespace
+
+FlatTree flatten(const Octree& tree) {
+  FlatTree f;
+  f.octree_size = tree.size();
+  f.brick_dim = tree.brick_dim();
+  f.colors = tree.color_palette();
+  const uint32_t dim = tree.brick_dim();
+  const size_t n3 = static_cast<size_t>(dim) * dim * dim;
+  const uint32_t words = occ_words_per_brick(dim);
+  const NodePool& pool = tree.nodes();
+  const auto& kids = tree.node_children();
+
+  // Breadth-first order over keys; `index_of` maps pool key -> record index.
+  std::vector<uint32_t> order;
+  std::unordered_map<uint32_t, uint32_t> index_of;
+  order.push_back(Octree::ROOT_NODE_KEY);
+  index_of.emplace(Octree::ROOT_NODE_KEY, 0);
+  size_t level_begin = 0;
+  while (level_begin < order.size()) {
+    const size_t level_stop = order.size();
+    for (size_t i = level_begin; i < level_stop; ++i) {
+      const uint32_t key = order[i];
+      const NodeContent& n = pool.get(key);
+      if (n.kind != NodeKind::Internal) continue;
+      for (uint32_t o = 0; o < 8; ++o) {
+        const uint32_t c = key < kids.size() ? kids[key].child(o) : EMPTY_MARKER;
+        if (!pool.key_is_valid(c) || index_of.count(c)) continue;
+        index_of.emplace(c, static_cast<uint32_t>(order.size()));
+        order.push_back(c);
+      }
+    }
+    f.level_end.push_back(static_cast<uint32_t>(level_stop));
+    level_begin = level_stop;
+  }
+
+  f.nodes.resize(order.size());
+  auto add_brick = [&](const std::vector<uint32_t>& voxels) {
+    const uint32_t slot = static_cast<uint32_t>(f.brick_pix.size() / (n3 ? n3 : 1));
+    f.brick_pix.insert(f.brick_pix.end(), voxels.begin(), voxels.end());
+    const size_t base = f.brick_occ.size();
+    f.brick_occ.resize(base + words, 0u);
+    for (size_t i = 0; i < n3; ++i)
+      if (!tree.pix_points_to_empty(voxels[i])) f.brick_occ[base + i / 32] |= 1u << (i % 32);
+    return slot;
+  };
+  auto encode_brick = [&](const Brick& b, uint32_t octant, NodeRec& r) {
+    uint32_t form = FORM_EMPTY;
+    uint32_t word = 0;
+    if (b.kind == BrickKind::Solid) {
+      form = FORM_SOLID;
+      word = b.solid;
+    } else if (b.kind == BrickKind::Parted) {
+      form = FORM_PARTED;
+      word = add_brick(b.parted);
+    }
+    r.meta |= form << (8u + 2u * octant);
+    r.child[octant] = word;
+  };
+
+  for (size_t i = 0; i < order.size(); ++i) {
+    const uint32_t key = order[i];
+    const NodeContent& n = pool.get(key);
+    NodeRec& r = f.nodes[i];
+    r = NodeRec{};
+    for (auto& c : r.child) c = INVALID_INDEX;
+    r.occ = tree.stored_occupied_bits(key);
+    switch (n.kind) {
+      case NodeKind::Nothing:
+        r.meta = KIND_NOTHING;
+        break;
+      case NodeKind::Internal:
+        r.meta = KIND_INTERNAL;
+        for (uint32_t o = 0; o < 8; ++o) {
+          const uint32_t c = key < kids.size() ? kids[key].child(o) : EMPTY_MARKER;
+          if (pool.key_is_valid(c)) r.child[o] = index_of.at(c);
+        }
+        break;
+      case NodeKind::Leaf:
+        r.meta = KIND_LEAF;
+        for (uint32_t o = 0; o < 8; ++o) encode_brick(n.bricks[o], o, r);
+        break;
+      case NodeKind::UniformLeaf:
+        r.meta = KIND_UNIFORM_LEAF;
+        encode_brick(n.bricks[0], 0, r);
+        break;
+    }
+  }
+  return f;
+}
+
+uint32_t flat_get(const FlatTree& f, const Octree& owner, uint32_t x, uint32_t y, uint32_t z) {
+  const V3f p{static_cast<float>(x), static_cast<float>(y), static_cast<float>(z)};
+  Cube bounds{v3u(0.0f), static_cast<float>(f.octree_size)};
+  if (!bound_contains(bounds, p)) return EMPTY_MARKER;
+  const uint32_t dim = f.brick_dim;
+  const size_t n3 = static_cast<size_t>(dim) * dim * dim;
+  auto parted_at = [&](uint32_t slot, Cube bb) {
+    const V3f rel = ((p - bb.min) * static_cast<float>(dim)) / bb.size;
+    const uint64_t ix = f2usize(roundf(floorf(rel.x)));
+    const uint64_t iy = f2usize(roundf(floorf(rel.y)));
+    const uint64_t iz = f2usize(roundf(floorf(rel.z)));
+    const uint64_t flat = ix + iy * dim + iz * dim * dim;
+    const uint32_t v = f.brick_pix[slot * n3 + flat];
+    return owner.pix_points_to_empty(v) ? EMPTY_MARKER : v;
+  };
+  uint32_t idx = 0;
+  for (;;) {
+    const NodeRec& r = f.nodes[idx];
+    switch (node_kind(r.meta)) {
+      case KIND_NOTHING:
+        return EMPTY_MARKER;
+      case KIND_LEAF: {
+        const uint32_t o = hash_region(p - bounds.min, bounds.size / 2.0f);
+        const uint32_t form = brick_form(r.meta, o);
+        if (form == FORM_EMPTY) return EMPTY_MARKER;
+        if (form == FORM_SOLID) return r.child[o];
+        return parted_at(r.child[o], child_bounds_for(bounds, o));
+      }
+      case KIND_UNIFORM_LEAF: {
+        const uint32_t form = brick_form(r.meta, 0);
+        if (form == FORM_EMPTY) return EMPTY_MARKER;
+        if (form == FORM_SOLID)
+          return owner.pix_points_to_empty(r.child[0]) ? EMPTY_MARKER : r.child[0];
+        return parted_at(r.child[0], bounds);
+      }
+      default: {
+        const uint32_t o = hash_region(p - bounds.min, bounds.size / 2.0f);
+        if (r.child[o] == INVALID_INDEX) return EMPTY_MARKER;
+        idx = r.child[o];
+        bounds = child_bounds_for(bounds, o);
+      }
+    }
+  }
+}
+
+void generate_ray_occupancy_lut(uint64_t out[64 * 8]) {
+  for (int i = 0; i < 64 * 8; ++i) out[i] = 0;
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y)
+      for (int z = 0; z < 4; ++z) {
+        const int pos = x + 4 * y + 16 * z;
+        for (int dx = -1; dx <= 1; dx += 2)
+          for (int dy = -1; dy <= 1; dy += 2)
+            for (int dz = -1; dz <= 1; dz += 2) {
+              const uint32_t dir = hash_direction(V3f{static_cast<float>(dx),
+                                                      static_cast<float>(dy),
+                                                      static_cast<float>(dz)});
+              auto span = [](int c, int d, int* lo, int* hi) {
+                int moved = c + d * 4;
+                moved = moved < 0 ? 0 : (moved > 3 ? 3 : moved);
+                *lo = moved < c ? moved : c;
+                *hi = moved > c ? moved : c;
+              };
+              int x0, x1, y0, y1, z0, z1;
+              span(x, dx, &x0, &x1);
+              span(y, dy, &y0, &y1);
+              span(z, dz, &z0, &z1);
+              uint64_t mask = 0;
+              for (int bx = x0; bx <= x1; ++bx)
+                for (int by = y0; by <= y1; ++by)
+                  for (int bz = z0; bz <= z1; ++bz) mask |= 1ull << (bx + 4 * by + 16 * bz);
+              out[pos * 8 + dir] = mask;
+            }
+      }
+}
+
+}  // namespace svo

@@ -1,0 +1,278 @@
+// Per-ray octree traversal over the device layout (layout.hpp).
+//
+// The control flow and every floating-point operation follow
+// get_by_ray_at_lod with MIP maps disabled (raytracing_on_cpu.rs:325-565),
+// probe_brick (:256-312) and traverse_brick (:156-252), so that one ray takes
+// the same path through the tree, bit for bit, as the reference CPU code:
+//   * a 4-entry ring stack whose push overwrites the oldest entry and whose
+//     pop past the retained entries restarts from the root after a 0.1 nudge
+//     along the ray (:19-82, :548-562);
+//   * the bitmap position is clamped to [1e-5, 4-1e-5] once per stack-loop
+//     iteration and then advanced by step*4/size without a clamp (:425-436,
+//     :510) - both are kept;
+//   * brick voxels are tested through the occupancy sidecar, which holds
+//     !pix_points_to_empty() for every voxel (node.rs:405-427).
+// Two guards exist that the reference lacks: an iteration bound (the
+// reference loop has no termination proof) and a bitmap-index range check
+// (the reference would index its LUT out of bounds and panic). Either sets a
+// flag on the ray; on well-formed inputs both stay clear.
+#pragma once
+
+#include "layout.hpp"
+#include "svo_math.hpp"
+
+namespace svo {
+
+constexpr uint32_t FLAG_ITERATION_GUARD = 1u;
+constexpr uint32_t FLAG_BITMAP_RANGE = 2u;
+constexpr uint32_t MAX_TRAVERSAL_STEPS = 1u << 22;
+
+struct TreeView {
+  const NodeRec* nodes;
+  const uint32_t* brick_pix;
+  const uint32_t* brick_occ;
+  const uint32_t* colors;
+  const uint64_t* ray_lut;  // RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, [pos*8 + dir]
+  float size;
+  uint32_t dim;
+  uint32_t occ_words;
+  uint32_t color_count;
+};
+
+struct CastResult {
+  uint32_t hit;
+  uint32_t pix;
+  uint32_t flags;
+  V3f impact;
+  V3f normal;
+  Cube cell;  // the voxel cell (Parted) or brick (Solid) that was hit
+};
+
+// Rust f32::clamp(min, max): NaN passes through
+SVO_HD inline float rclamp(float x, float lo, float hi) {
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  return x;
+}
+
+// BITMAP_INDEX_LUT[floor(x) as usize][floor(y)][floor(z)] = x + 4y + 16z
+SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
+  uint64_t x = f2usize(floorf(bp.x));
+  uint64_t y = f2usize(floorf(bp.y));
+  uint64_t z = f2usize(floorf(bp.z));
+  if (x > 3 || y > 3 || z > 3) {
+    *flags |= FLAG_BITMAP_RANGE;
+    x = x > 3 ? 3 : x;
+    y = y > 3 ? 3 : y;
+    z = z > 3 ? 3 : z;
+  }
+  return static_cast<uint32_t>(x + 4 * y + 16 * z);
+}
+
+// probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
+SVO_HD inline bool probe_brick(const TreeView& t, uint32_t form, uint32_t word, Cube bb, V3f d,
+                               V3f sig, V3f scale, V3f* p, CastResult* r) {
+  if (form == FORM_EMPTY) return false;
+  if (form == FORM_SOLID) {
+    r->pix = word;
+    r->impact = *p;
+    r->normal = cube_impact_normal(bb, *p);
+    r->cell = bb;
+    return true;
+  }
+  const int dim = static_cast<int>(t.dim);
+  const float fdim = static_cast<float>(t.dim);
+  const V3f pib = ((*p - bb.min) * fdim) / bb.size;
+  auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
+  int ix = clampi(f2i32(pib.x));
+  int iy = clampi(f2i32(pib.y));
+  int iz = clampi(f2i32(pib.z));
+  int flat = ix + iy * dim + iz * dim * dim;
+  const float unit = bb.size / fdim;
+  Cube cb{bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
+                       unit,
+          unit};
+  const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
+  V3f step{0.0f, 0.0f, 0.0f};
+  for (int guard = 0; guard < 4 * dim + 16; ++guard) {
+    if (ix < 0 || ix >= dim || iy < 0 || iy >= dim || iz < 0 || iz >= dim) return false;
+    flat += f2i32(step.x) * 1 + f2i32(step.y) * dim + f2i32(step.z) * (dim * dim);
+    if ((occ[static_cast<uint32_t>(flat) >> 5] >> (static_cast<uint32_t>(flat) & 31u)) & 1u) {
+      const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
+      r->pix = t.brick_pix[static_cast<uint64_t>(word) * n3 + static_cast<uint32_t>(flat)];
+      const Cube hb{bb.min + (V3f{static_cast<float>(ix), static_cast<float>(iy),
+                                  static_cast<float>(iz)} *
+                              bb.size) /
+                                 fdim,
+                    bb.size / fdim};
+      r->impact = *p;
+      r->normal = cube_impact_normal(hb, *p);
+      r->cell = hb;
+      return true;
+    }
+    step = dda_step(d, sig, p, cb, scale);
+    cb.min = cb.min + step * unit;
+    ix += f2i32(roundf(step.x));
+    iy += f2i32(roundf(step.y));
+    iz += f2i32(roundf(step.z));
+  }
+  r->flags |= FLAG_ITERATION_GUARD;
+  return false;
+}
+
+// get_by_ray_at_lod(ray, f32::MAX) with MIPs disabled
+SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
+  CastResult r;
+  r.hit = 0;
+  r.pix = EMPTY_MARKER;
+  r.flags = 0;
+  r.impact = V3f{0.0f, 0.0f, 0.0f};
+  r.normal = V3f{0.0f, 0.0f, 0.0f};
+  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
+
+  const V3f scale = dda_scale_factors(d);
+  const uint32_t dir_idx = hash_direction(d);
+  const V3f sig{rsignum(d.x), rsignum(d.y), rsignum(d.z)};
+  const Cube root{v3u(0.0f), t.size};
+  const float lo = 0.00001f;
+  const float hi = 4.0f - 0.00001f;
+
+  uint32_t stk[4] = {0u, 0u, 0u, 0u};
+  uint32_t head = 0, count = 0;
+  Cube bounds = root;
+  V3f p;
+  uint32_t target;
+  {
+    float tt = 0.0f;
+    const int ir = intersect_ray(root, o, d, &tt);
+    if (ir != 0) {
+      p = o + d * (ir == 2 ? tt : 0.0f);
+      target = hash_region(p - root.min, root.size / 2.0f);
+    } else {
+      p = o;
+      target = OOB_OCTANT;
+    }
+  }
+  uint32_t steps = 0;
+  while (target != OOB_OCTANT) {
+    bounds = root;
+    head = (head + 1u) & 3u;
+    count = count < 4u ? count + 1u : 4u;
+    stk[head] = 0u;
+    while (count != 0u) {
+      if (++steps > MAX_TRAVERSAL_STEPS) {
+        r.flags |= FLAG_ITERATION_GUARD;
+        return r;
+      }
+      const NodeRec& n = t.nodes[stk[head]];
+      const uint64_t occ = n.occ;
+      const uint32_t meta = n.meta;
+      const uint32_t kind = node_kind(meta);
+      bool backtrack = kind == KIND_UNIFORM_LEAF;
+      if (target != OOB_OCTANT) {
+        if (kind == KIND_UNIFORM_LEAF) {
+          if (probe_brick(t, brick_form(meta, 0), n.child[0], bounds, d, sig, scale, &p, &r)) {
+            r.hit = 1;
+            return r;
+          }
+          backtrack = true;
+        } else if (kind == KIND_LEAF) {
+          if (probe_brick(t, brick_form(meta, target), n.child[target],
+                          child_bounds_for(bounds, target), d, sig, scale, &p, &r)) {
+            r.hit = 1;
+            return r;
+          }
+        }
+      }
+      V3f bp = ((p - bounds.min) * 4.0f) / bounds.size;
+      bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
+      uint32_t flat = bitmap_index(bp, &r.flags);
+      if (backtrack || target == OOB_OCTANT || occ == 0ull ||
+          (occ & t.ray_lut[flat * 8u + dir_idx]) == 0ull) {
+        // POP (:445-475)
+        head = (head + 3u) & 3u;
+        count -= 1u;
+        if (count != 0u) {
+          const V3f center = bounds.min + v3u(bounds.size / 2.0f);
+          const float s2 = bounds.size * 2.0f;
+          const V3f pmin =
+              bounds.min - V3f{fmodf(bounds.min.x, s2), fmodf(bounds.min.y, s2),
+                               fmodf(bounds.min.z, s2)};
+          const uint32_t from = hash_region(center - pmin, bounds.size);
+          const V3f st = dda_step(d, sig, &p, bounds, scale);
+          target = step_octant(from, st);
+          bounds.size *= 2.0f;
+          bounds.min = pmin;
+        }
+        continue;
+      }
+      Cube tb = child_bounds_for(bounds, target);
+      uint32_t tck = kind == KIND_INTERNAL ? n.child[target] : INVALID_INDEX;
+      if (tck != INVALID_INDEX && (occ & octant_bitmap_mask(target)) != 0ull) {
+        // PUSH (:484-492)
+        bounds = tb;
+        target = hash_region(p - tb.min, tb.size / 2.0f);
+        head = (head + 1u) & 3u;
+        count = count < 4u ? count + 1u : 4u;
+        stk[head] = tck;
+      } else {
+        // ADVANCE (:497-544)
+        for (;;) {
+          if (++steps > MAX_TRAVERSAL_STEPS) {
+            r.flags |= FLAG_ITERATION_GUARD;
+            return r;
+          }
+          const V3f st = dda_step(d, sig, &p, tb, scale);
+          target = step_octant(target, st);
+          if (target != OOB_OCTANT) {
+            tb = child_bounds_for(bounds, target);
+            tck = kind == KIND_INTERNAL ? n.child[target] : INVALID_INDEX;
+            bp = bp + (st * 4.0f) / bounds.size;
+            flat = bitmap_index(bp, &r.flags);
+          }
+          if (target == OOB_OCTANT ||
+              (tck != INVALID_INDEX && (occ & octant_bitmap_mask(target)) != 0ull &&
+               (t.ray_lut[flat * 8u + dir_idx] & occ) != 0ull) ||
+              (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY))
+            break;
+        }
+      }
+    }
+    // restart from the root (:551-562)
+    p = p + d * 0.1f;
+    target = (p.x < t.size && p.y < t.size && p.z < t.size && p.x > 0.0f && p.y > 0.0f &&
+              p.z > 0.0f)
+                 ? hash_region(p, t.size / 2.0f)
+                 : OOB_OCTANT;
+  }
+  return r;
+}
+
+// Voxel coordinate of a hit: floor(impact) clamped into the hit cell.
+SVO_HD inline uint32_t hit_voxel_axis(float p, float cmin, float csize) {
+  float f = floorf(p);
+  const float hi = cmin + csize - 1.0f;
+  f = f < cmin ? cmin : (f > hi ? hi : f);
+  return f2u32(f);
+}
+
+// Ray of pixel (x, y) of a pinhole viewport (y = 0 is the bottom row), as the
+// reference's CPU harness builds it (examples/cpu_render.rs:104-114) with the
+// ray through the pixel centre.
+struct Camera {
+  V3f origin;
+  V3f bottom_left;
+  V3f right;
+  V3f up;
+  float pixel_w;
+  float pixel_h;
+};
+
+SVO_HD inline V3f camera_ray_dir(const Camera& c, uint32_t x, uint32_t y) {
+  const float fx = static_cast<float>(x) + 0.5f;
+  const float fy = static_cast<float>(y) + 0.5f;
+  const V3f glass = (c.bottom_left + (c.right * fx) * c.pixel_w) + (c.up * fy) * c.pixel_h;
+  return normalized(glass - c.origin);
+}
+
+}  // namespace svo

@@ -1,0 +1,370 @@
+"""Python mirror of the reference's octree API over libsvo.
+
+`Octree` mirrors shocovox-rs `Octree<u32>` (src/octree/mod.rs, update/*.rs):
+new / insert / insert_at_lod / update / clear / clear_at_lod / get /
+get_size, with `OctreeEntry` values. `DeviceTree` is the device-resident form
+that answers `get_by_ray` for batches of rays on the GPU
+(src/raytracing/raytracing_on_cpu.rs:316) and renders viewports
+(the bevy `OctreeGPUView`, src/raytracing/bevy/types.rs:55-134).
+Device buffers are torch tensors; the traversal itself runs in libsvo's HIP
+kernels, and nothing here computes on the CPU in its place.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+EMPTY, VISUAL, INFORMATIVE, COMPLEX = 0, 1, 2, 3
+EMPTY_MARKER = 0xFFFFFFFF
+
+
+def albedo(r: int, g: int, b: int, a: int) -> int:
+    """Packs an Albedo like Albedo::from(u32) unpacks it (detail.rs:92-105)."""
+    return ((r & 0xFF) << 24) | ((g & 0xFF) << 16) | ((b & 0xFF) << 8) | (a & 0xFF)
+
+
+def albedo_rgba(v: int) -> tuple[int, int, int, int]:
+    return ((v >> 24) & 0xFF, (v >> 16) & 0xFF, (v >> 8) & 0xFF, v & 0xFF)
+
+
+@dataclass(frozen=True)
+class OctreeEntry:
+    """OctreeEntry (src/octree/types.rs:23-36)."""
+
+    kind: int = EMPTY
+    albedo: int = 0
+    data: int = 0
+
+    @staticmethod
+    def visual(albedo_u32: int) -> "OctreeEntry":
+        return OctreeEntry(VISUAL, albedo_u32 & 0xFFFFFFFF, 0)
+
+    @staticmethod
+    def informative(data: int) -> "OctreeEntry":
+        return OctreeEntry(INFORMATIVE, 0, data & 0xFFFFFFFF)
+
+    @staticmethod
+    def complex(albedo_u32: int, data: int) -> "OctreeEntry":
+        return OctreeEntry(COMPLEX, albedo_u32 & 0xFFFFFFFF, data & 0xFFFFFFFF)
+
+    def is_none(self) -> bool:
+        """OctreeEntry::is_none (src/octree/mod.rs:97-104)."""
+        if self.kind == EMPTY:
+            return True
+        if self.kind == VISUAL:
+            return (self.albedo & 0xFF) == 0
+        if self.kind == INFORMATIVE:
+            return self.data == 0
+        return (self.albedo & 0xFF) == 0 and self.data == 0
+
+    def is_some(self) -> bool:
+        return not self.is_none()
+
+    def to_c(self) -> N.Entry:
+        return N.Entry(self.kind, self.albedo, self.data)
+
+    @staticmethod
+    def from_c(e: N.Entry) -> "OctreeEntry":
+        return OctreeEntry(int(e.kind), int(e.albedo), int(e.data))
+
+
+def voxel_data(d: int) -> OctreeEntry:
+    """The reference's `voxel_data!` macro (src/octree/mod.rs:62-70)."""
+    return OctreeEntry.informative(d)
+
+
+class OctreeError(N.SvoError):
+    pass
+
+
+def _check(status: int) -> None:
+    if status != N.SVO_OK:
+        msg = N.lib().svo_last_error()
+        raise OctreeError(status, msg.decode() if msg else "")
+
+
+class Octree:
+    """Host octree, Octree<u32> of the reference."""
+
+    def __init__(self, size: int, brick_dim: int, _handle: int | None = None):
+        lib = N.lib()
+        if _handle is None:
+            h = C.c_void_p()
+            _check(lib.svo_octree_new(size, brick_dim, C.byref(h)))
+            _handle = h.value
+        self._h = _handle
+
+    @classmethod
+    def new(cls, size: int, brick_dim: int) -> "Octree":
+        return cls(size, brick_dim)
+
+    @classmethod
+    def scene(cls, kind: int, size: int, brick_dim: int, seed: int) -> "Octree":
+        """Builds one of the seeded benchmark scenes (see include/svo.h)."""
+        h = C.c_void_p()
+        _check(N.lib().svo_scene_build(kind, size, brick_dim, seed, C.byref(h)))
+        return cls(0, 0, _handle=h.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().svo_octree_free(h)
+            self._h = None
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    def get_size(self) -> int:
+        return int(N.lib().svo_octree_get_size(self._h))
+
+    @property
+    def brick_dim(self) -> int:
+        return int(N.lib().svo_octree_get_brick_dim(self._h))
+
+    def set_auto_simplify(self, enabled: bool) -> None:
+        _check(N.lib().svo_octree_set_auto_simplify(self._h, int(enabled)))
+
+    def insert(self, pos, entry: OctreeEntry) -> None:
+        e = entry.to_c()
+        _check(N.lib().svo_octree_insert(self._h, pos[0], pos[1], pos[2], C.byref(e)))
+
+    def insert_at_lod(self, pos, size: int, entry: OctreeEntry) -> None:
+        e = entry.to_c()
+        _check(N.lib().svo_octree_insert_at_lod(self._h, pos[0], pos[1], pos[2], size,
+                                                 C.byref(e)))
+
+    def update(self, pos, entry: OctreeEntry) -> None:
+        e = entry.to_c()
+        _check(N.lib().svo_octree_update(self._h, pos[0], pos[1], pos[2], C.byref(e)))
+
+    def clear(self, pos) -> None:
+        _check(N.lib().svo_octree_clear(self._h, pos[0], pos[1], pos[2]))
+
+    def clear_at_lod(self, pos, size: int) -> None:
+        _check(N.lib().svo_octree_clear_at_lod(self._h, pos[0], pos[1], pos[2], size))
+
+    def get(self, pos) -> OctreeEntry:
+        e = N.Entry()
+        _check(N.lib().svo_octree_get(self._h, pos[0], pos[1], pos[2], C.byref(e)))
+        return OctreeEntry.from_c(e)
+
+    def insert_many(self, xyz: np.ndarray, entries: np.ndarray) -> None:
+        """insert() for each row of xyz (n,3 u32) with entries (n,3 u32: kind, albedo, data)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32)
+        entries = np.ascontiguousarray(entries, dtype=np.uint32)
+        _check(N.lib().svo_octree_insert_many(self._h, xyz.ctypes.data, entries.ctypes.data,
+                                              len(xyz)))
+
+    def clear_many(self, xyz: np.ndarray) -> None:
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32)
+        _check(N.lib().svo_octree_clear_many(self._h, xyz.ctypes.data, len(xyz)))
+
+    def get_many(self, xyz: np.ndarray) -> np.ndarray:
+        """get() for each row; returns (n,3) u32 rows of (kind, albedo, data)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32)
+        out = np.zeros((len(xyz), 3), dtype=np.uint32)
+        _check(N.lib().svo_octree_get_many(self._h, xyz.ctypes.data, len(xyz), out.ctypes.data))
+        return out
+
+    def export_model(self) -> dict:
+        """Raw dump of the node model (see svo_octree_export)."""
+        lib = N.lib()
+        s = N.ModelSizes()
+        _check(lib.svo_octree_export(self._h, C.byref(s), None))
+        dim = s.brick_dim
+        n = s.pool_len
+        out = {
+            "octree_size": s.octree_size, "brick_dim": dim,
+            "reserved": np.zeros(n, np.uint8), "content_kind": np.zeros(n, np.uint8),
+            "internal_bits": np.zeros(n, np.uint64), "children_kind": np.zeros(n, np.uint8),
+            "children": np.zeros(n * 8, np.uint32), "children_bitmap": np.zeros(n, np.uint64),
+            "brick_kind": np.zeros(n * 8, np.uint8), "brick_value": np.zeros(n * 8, np.uint32),
+            "voxels": np.zeros(max(1, s.parted_count * dim ** 3), np.uint32),
+            "colors": np.zeros(max(1, s.color_count), np.uint32),
+            "data": np.zeros(max(1, s.data_count), np.uint32),
+        }
+        d = N.ModelDump(*(out[k].ctypes.data for k in (
+            "reserved", "content_kind", "internal_bits", "children_kind", "children",
+            "children_bitmap", "brick_kind", "brick_value", "voxels", "colors", "data")))
+        _check(lib.svo_octree_export(self._h, C.byref(s), C.byref(d)))
+        out["pool_len"] = n
+        out["parted_count"] = s.parted_count
+        out["color_count"] = s.color_count
+        out["data_count"] = s.data_count
+        return out
+
+    def flatten(self) -> "FlatTree":
+        h = C.c_void_p()
+        _check(N.lib().svo_octree_flatten(self._h, C.byref(h)))
+        return FlatTree(h.value, self)
+
+
+NODE_DTYPE = np.dtype([("occ", "<u8"), ("meta", "<u4"), ("reserved0", "<u4"),
+                       ("child", "<u4", (8,)), ("reserved1", "<u4", (4,))])
+
+
+class FlatTree:
+    """Flattened device layout of an Octree, still on the host."""
+
+    def __init__(self, handle: int, owner: Octree):
+        self._h = handle
+        self._owner = owner
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().svo_flat_free(h)
+            self._h = None
+
+    def desc(self) -> N.TreeDesc:
+        d = N.TreeDesc()
+        _check(N.lib().svo_flat_desc(self._h, C.byref(d)))
+        return d
+
+    def arrays(self) -> dict:
+        d = self.desc()
+        n3 = d.brick_dim ** 3
+        words = (n3 + 31) // 32
+
+        def view(ptr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr)
+            return np.frombuffer(buf, dtype=dtype).copy()
+
+        return {
+            "octree_size": d.octree_size, "brick_dim": d.brick_dim,
+            "top_level_nodes": d.top_level_nodes,
+            "nodes": view(d.nodes, d.node_count, NODE_DTYPE),
+            "brick_pix": view(d.brick_pix, d.brick_count * n3, np.uint32),
+            "brick_occ": view(d.brick_occ, d.brick_count * words, np.uint32),
+            "colors": view(d.colors, d.color_count, np.uint32),
+        }
+
+    def get(self, pos) -> OctreeEntry:
+        e = N.Entry()
+        _check(N.lib().svo_flat_get(self._h, self._owner.handle, pos[0], pos[1], pos[2],
+                                    C.byref(e)))
+        return OctreeEntry.from_c(e)
+
+
+@dataclass
+class Viewport:
+    """Viewport (src/raytracing/bevy/types.rs:55-71)."""
+
+    origin: tuple
+    direction: tuple
+    frustum: tuple  # glass width, glass height, depth
+    fov: float  # distance of the glass from the eye
+
+    def to_c(self) -> N.Viewport:
+        v = N.Viewport()
+        for i in range(3):
+            v.origin[i] = float(self.origin[i])
+            v.direction[i] = float(self.direction[i])
+            v.frustum[i] = float(self.frustum[i])
+        v.fov = float(self.fov)
+        return v
+
+
+HIT_FIELDS = ("hit", "pix", "albedo", "voxel", "distance", "impact", "normal", "flags")
+
+
+def _torch():
+    import torch  # device memory and streams only
+
+    return torch
+
+
+def alloc_hits(n: int, device="cuda", fields=HIT_FIELDS) -> dict:
+    """Device output buffers for n rays."""
+    torch = _torch()
+    shapes = {"hit": (n,), "pix": (n,), "albedo": (n,), "voxel": (n, 3), "distance": (n,),
+              "impact": (n, 3), "normal": (n, 3), "flags": (n,)}
+    dtypes = {"hit": torch.int32, "pix": torch.int32, "albedo": torch.int32,
+              "voxel": torch.int32, "distance": torch.float32, "impact": torch.float32,
+              "normal": torch.float32, "flags": torch.int32}
+    return {k: torch.empty(shapes[k], dtype=dtypes[k], device=device) for k in fields}
+
+
+def _hits_struct(bufs: dict) -> N.Hits:
+    h = N.Hits()
+    for k in HIT_FIELDS:
+        t = bufs.get(k)
+        setattr(h, k, t.data_ptr() if t is not None else None)
+    return h
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        torch = _torch()
+        return torch.cuda.current_stream().cuda_stream or None
+    return getattr(stream, "cuda_stream", stream) or None
+
+
+class DeviceTree:
+    """A tree resident in HBM; answers get_by_ray for batches of rays."""
+
+    def __init__(self, octree: Octree | None = None, desc: N.TreeDesc | None = None):
+        lib = N.lib()
+        h = C.c_void_p()
+        if octree is not None:
+            _check(lib.svo_tree_from_octree(octree.handle, C.byref(h)))
+        elif desc is not None:
+            _check(lib.svo_tree_create(C.byref(desc), C.byref(h)))
+        else:
+            raise ValueError("need an octree or a tree description")
+        self._h = h.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().svo_tree_destroy(h)
+            self._h = None
+
+    @property
+    def device_bytes(self) -> int:
+        return int(N.lib().svo_tree_device_bytes(self._h))
+
+    def update(self, octree: Octree, stream=None) -> None:
+        """Re-syncs the device copy with the host octree (OctreeGPUView::reload)."""
+        _check(N.lib().svo_tree_update(self._h, octree.handle, _stream_ptr(stream)))
+
+    def cast_into(self, rays, n: int, bufs: dict, stream=None) -> None:
+        hs = _hits_struct(bufs)
+        _check(N.lib().svo_cast(self._h, rays.data_ptr(), n, C.byref(hs), _stream_ptr(stream)))
+
+    def cast(self, rays, fields=HIT_FIELDS, stream=None) -> dict:
+        """get_by_ray for every row of `rays` (n x 6 f32 device tensor)."""
+        torch = _torch()
+        rays = rays.to(dtype=torch.float32).contiguous()
+        n = rays.shape[0]
+        bufs = alloc_hits(n, rays.device, fields)
+        self.cast_into(rays, n, bufs, stream)
+        return bufs
+
+    def render_into(self, vp: Viewport, w: int, h: int, bufs: dict, stream=None) -> None:
+        hs = _hits_struct(bufs)
+        v = vp.to_c()
+        _check(N.lib().svo_render(self._h, C.byref(v), w, h, C.byref(hs), _stream_ptr(stream)))
+
+    def render(self, vp: Viewport, w: int, h: int, fields=HIT_FIELDS, device="cuda",
+               stream=None) -> dict:
+        bufs = alloc_hits(w * h, device, fields)
+        self.render_into(vp, w, h, bufs, stream)
+        return bufs
+
+
+def set_timing(enabled: bool) -> None:
+    N.lib().svo_set_timing(int(enabled))
+
+
+def collect_kernel_ms(cap: int = 4096) -> list:
+    buf = (C.c_float * cap)()
+    n = N.lib().svo_collect_kernel_ms(buf, cap)
+    return [buf[i] for i in range(n)]

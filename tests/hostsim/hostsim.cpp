@@ -1,0 +1,85 @@
+// Test-only build of the product's device traversal (traverse.hpp) for the
+// host CPU, so the kernel's logic can be compared with the oracle on
+// machines without a GPU. It is not part of libsvo and no product code path
+// loads it.
+#include <cstdint>
+
+#include "../../shocovox_amd/csrc/traverse.hpp"
+
+extern "C" {
+
+struct hs_tree {
+  const svo::NodeRec* nodes;
+  const uint32_t* brick_pix;
+  const uint32_t* brick_occ;
+  const uint32_t* colors;
+  const uint64_t* ray_lut;
+  float size;
+  uint32_t dim;
+  uint32_t occ_words;
+  uint32_t color_count;
+};
+
+struct hs_hit {
+  uint32_t hit, pix, albedo, flags;
+  uint32_t voxel[3];
+  float distance;
+  float impact[3];
+  float normal[3];
+};
+
+void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out) {
+  svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut,
+                  t->size,  t->dim,       t->occ_words, t->color_count};
+  for (uint64_t i = 0; i < n; ++i) {
+    const float* r = rays + 6 * i;
+    const svo::V3f o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+    const svo::CastResult c = svo::cast_ray(v, o, d);
+    hs_hit& h = out[i];
+    h.hit = c.hit;
+    h.flags = c.flags;
+    h.pix = c.hit ? c.pix : 0xFFFFFFFFu;
+    const uint32_t ci = c.pix & 0xFFFFu;
+    h.albedo = (c.hit && ci < 0xFFFFu && ci < t->color_count) ? t->colors[ci] : 0u;
+    if (c.hit) {
+      h.voxel[0] = svo::hit_voxel_axis(c.impact.x, c.cell.min.x, c.cell.size);
+      h.voxel[1] = svo::hit_voxel_axis(c.impact.y, c.cell.min.y, c.cell.size);
+      h.voxel[2] = svo::hit_voxel_axis(c.impact.z, c.cell.min.z, c.cell.size);
+      h.distance = svo::length(c.impact - o);
+    } else {
+      h.voxel[0] = h.voxel[1] = h.voxel[2] = 0xFFFFFFFFu;
+      h.distance = __builtin_inff();
+    }
+    h.impact[0] = c.impact.x;
+    h.impact[1] = c.impact.y;
+    h.impact[2] = c.impact.z;
+    h.normal[0] = c.normal.x;
+    h.normal[1] = c.normal.y;
+    h.normal[2] = c.normal.z;
+  }
+}
+
+void hs_viewport_rays(const float* vp, uint32_t w, uint32_t h, float* rays) {
+  using svo::V3f;
+  const V3f origin{vp[0], vp[1], vp[2]}, dir{vp[3], vp[4], vp[5]}, up{0.f, 1.f, 0.f};
+  svo::Camera cam;
+  cam.origin = origin;
+  cam.up = up;
+  cam.right = svo::normalized(svo::cross(up, dir));
+  cam.pixel_w = vp[6] / static_cast<float>(w);
+  cam.pixel_h = vp[7] / static_cast<float>(h);
+  cam.bottom_left = ((origin + dir * vp[9]) - up * (vp[7] / 2.0f)) - cam.right * (vp[6] / 2.0f);
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t x = 0; x < w; ++x) {
+      const V3f d = svo::camera_ray_dir(cam, x, y);
+      float* r = rays + 6 * (static_cast<uint64_t>(y) * w + x);
+      r[0] = origin.x;
+      r[1] = origin.y;
+      r[2] = origin.z;
+      r[3] = d.x;
+      r[4] = d.y;
+      r[5] = d.z;
+    }
+}
+
+}  // extern "C"
