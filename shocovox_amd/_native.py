@@ -113,6 +113,13 @@ def lib() -> C.CDLL:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m shocovox_amd.build`")
+    # torch bundles its own HIP runtime with the same SONAME (libamdhip64.so.7).
+    # Loading torch first makes libsvo bind to that copy, so device pointers
+    # from torch tensors and libsvo's kernels share one runtime per process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     handle = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIGS.items():
         fn = getattr(handle, name)

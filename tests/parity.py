@@ -1,0 +1,51 @@
+"""Comparison of GPU results with the oracle, field by field.
+
+The bar: hit flag, palette index, albedo, voxel coordinate and flags equal
+exactly; impact point, normal and distance equal bit for bit as well (the
+kernel follows the reference's f32 operation sequence), with distance also
+reported against the 1e-4 voxel-unit tolerance."""
+from __future__ import annotations
+
+import numpy as np
+
+DIST_TOL = 1e-4  # voxel units
+
+EXACT_FIELDS = ("hit", "pix", "albedo", "voxel", "flags")
+FLOAT_FIELDS = ("distance", "impact", "normal")
+
+
+def gpu_to_numpy(bufs: dict) -> dict:
+    out = {}
+    for k, t in bufs.items():
+        a = t.detach().cpu().numpy()
+        if a.dtype == np.int32:
+            a = a.view(np.uint32)
+        out[k] = a
+    return out
+
+
+def compare(expected: np.ndarray, got: dict) -> dict:
+    """Mismatch counts per field (rays that differ anywhere in the field)."""
+    n = len(expected)
+    res = {"rays": n}
+    for f in EXACT_FIELDS + FLOAT_FIELDS:
+        if f not in got:
+            continue
+        e = np.ascontiguousarray(expected[f]).reshape(n, -1)
+        g = np.ascontiguousarray(got[f]).reshape(n, -1)
+        if e.dtype.kind == "f":
+            e = e.view(np.uint32)
+            g = np.ascontiguousarray(g, dtype=np.float32).view(np.uint32)
+        res[f] = int((e != g).any(axis=1).sum())
+    if "distance" in got:
+        hit = expected["hit"] == 1
+        d = np.abs(expected["distance"][hit].astype(np.float64) - got["distance"][hit].astype(np.float64))
+        res["distance_over_tol"] = int((d > DIST_TOL).sum())
+    return res
+
+
+def assert_parity(expected: np.ndarray, got: dict, label: str = "") -> dict:
+    res = compare(expected, got)
+    bad = {k: v for k, v in res.items() if k != "rays" and v}
+    assert not bad, f"{label}: mismatching rays per field {bad} of {res['rays']}"
+    return res

@@ -1,0 +1,175 @@
+"""GPU parity: the HIP traversal against the oracle, through the C ABI.
+
+Every test prints what it is about to run (workload, seed, ray count) before
+touching the GPU, so a failing case is named even if the run dies.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from kat_trees import fixed_kats, make_edge_ray_point_to, make_ray_point_to, random_fill
+from parity import assert_parity, gpu_to_numpy
+from shocovox_amd import DeviceTree, Octree, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
+def _cast(tree: Octree, rays: np.ndarray, dev) -> dict:
+    import torch
+    d = DeviceTree(tree)
+    t = torch.from_numpy(np.ascontiguousarray(rays, np.float32).reshape(-1, 6)).to(dev)
+    bufs = d.cast(t)
+    torch.cuda.synchronize()
+    return gpu_to_numpy(bufs)
+
+
+def _oracle(tree: Octree) -> oracle.OracleTree:
+    return oracle.OracleTree(tree.export_model())
+
+
+def test_kat_fixed_rays(gpu_device):
+    """The 17 fixed-ray known-answer tests of raytracing/tests.rs on the GPU."""
+    for name, (builder, r, expect) in fixed_kats().items():
+        _say(f"KAT {name}")
+        tree = builder()
+        got = _cast(tree, r[None, :], gpu_device)
+        exp = _oracle(tree).cast(r[None, :])
+        assert_parity(exp, got, name)
+        hit = bool(got["hit"][0])
+        kind = expect[0]
+        if kind == "some":
+            assert hit, name
+        elif kind == "none":
+            assert not hit, name
+        elif kind == "data":
+            # voxel_data!(&5) is Informative: no colour index, data index set
+            assert hit and got["albedo"][0] == 0, name
+            assert (got["pix"][0] & 0xFFFF) == 0xFFFF and (got["pix"][0] >> 16) != 0xFFFF, name
+        elif kind == "albedo":
+            assert hit and got["albedo"][0] == expect[1], name
+        elif kind == "albedo_normal":
+            assert hit and got["albedo"][0] == expect[1], name
+            assert tuple(got["normal"][0]) == expect[2], name
+        elif kind == "albedo_normal_len":
+            assert hit and got["albedo"][0] == expect[1], name
+            assert np.linalg.norm(got["normal"][0]) < expect[2], name
+        elif kind == "none_or_data":
+            assert (not hit) or (got["pix"][0] & 0xFFFF) == 0xFFFF, name
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_kat_random_trees(gpu_device, seed):
+    """test_get_by_ray_from_outside / _where_dim_is_2 / _from_edge / _from_inside
+    (raytracing/tests.rs:144-251) with seeded generators."""
+    rng = np.random.default_rng(seed)
+    cases = [
+        ("from_outside", *random_fill(4, 1, rng), make_ray_point_to, 0.0),
+        ("from_outside_dim2", *random_fill(4, 2, rng), make_ray_point_to, 0.0),
+        ("from_edge", *random_fill(8, 1, rng, zs=(1, 2, 3)), make_edge_ray_point_to, 0.1),
+        ("from_inside", *random_fill(16, 1, rng, zs=(1, 2, 3)), make_ray_point_to, 0.0),
+    ]
+    for name, tree, filled, maker, off in cases:
+        if not filled:
+            continue
+        rays = np.stack([maker(np.array(p, np.float32) + np.float32(off), rng) for p in filled])
+        _say(f"random KAT {name} seed={seed} rays={len(rays)}")
+        got = _cast(tree, rays, gpu_device)
+        exp = _oracle(tree).cast(rays)
+        assert_parity(exp, got, f"{name}/{seed}")
+        assert got["hit"].all(), f"{name}/{seed}: every ray at a filled voxel must hit"
+        assert ((got["pix"] & 0xFFFF) == 0xFFFF).all()
+
+
+def test_plain_render_all_rays(gpu_device):
+    """Every primary ray of the 128^3 / dim 8 workload, generated on the GPU."""
+    import torch
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    _say(f"plain render {cfg.width}x{cfg.height}")
+    d = DeviceTree(tree)
+    bufs = d.render(vp, cfg.width, cfg.height)
+    torch.cuda.synchronize()
+    got = gpu_to_numpy(bufs)
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    exp = _oracle(tree).cast(rays)
+    res = assert_parity(exp, got, "plain render")
+    assert exp["hit"].sum() > 1000
+    _say(f"plain render parity {res}")
+
+
+def test_plain_cast_matches_render(gpu_device):
+    """svo_cast on host-generated rays equals svo_render on the same viewport."""
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=2)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    _say("plain cast")
+    got = _cast(tree, rays, gpu_device)
+    exp = _oracle(tree).cast(rays)
+    assert_parity(exp, got, "plain cast")
+
+
+def test_caves_pose_render(gpu_device):
+    """One full 1920x1080 pose of the 512^3 / dim 16 workload."""
+    import torch
+    cfg = W.CAVES
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[3]
+    _say(f"caves render pose 3 {cfg.width}x{cfg.height}")
+    d = DeviceTree(tree)
+    bufs = d.render(vp, cfg.width, cfg.height)
+    torch.cuda.synchronize()
+    got = gpu_to_numpy(bufs)
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    exp = _oracle(tree).cast(rays, threads=16)
+    res = assert_parity(exp, got, "caves pose")
+    _say(f"caves parity {res}")
+
+
+def test_caves_incoherent(gpu_device):
+    """Incoherent rays (origins in the volume, directions on S^2) into 512^3."""
+    cfg = W.CAVES
+    tree = W.build_scene(cfg, seed=3)
+    rays = W.incoherent_rays(cfg.size, 200_000, seed=5)
+    _say("caves incoherent 200000")
+    got = _cast(tree, rays, gpu_device)
+    exp = _oracle(tree).cast(rays, threads=16)
+    assert_parity(exp, got, "caves incoherent")
+
+
+def test_axis_aligned_and_degenerate_rays(gpu_device):
+    """Rays along axes and planes (inf/NaN scale factors, signed zeros)."""
+    tree = W.build_scene(W.PLAIN, seed=1)
+    rays = []
+    for o in [(64.5, 200.0, 64.5), (-10.0, 20.0, 30.0), (64.0, 64.0, -5.0), (0.0, 0.0, 0.0),
+              (128.0, 40.0, 128.0), (20.0, 127.99, 20.0)]:
+        for d in [(0, -1, 0), (1, 0, 0), (0, 0, 1), (-1, 0, 0), (0, 1, 0), (0, 0, -1),
+                  (0.70710677, -0.70710677, 0), (0, -0.70710677, 0.70710677),
+                  (-0.0, -1.0, 0.0), (0.57735026, -0.57735026, 0.57735026)]:
+            rays.append([*o, *d])
+    rays = np.array(rays, np.float32)
+    _say(f"axis-aligned rays {len(rays)}")
+    got = _cast(tree, rays, gpu_device)
+    exp = _oracle(tree).cast(rays)
+    assert_parity(exp, got, "axis aligned")
+
+
+def test_empty_inputs(gpu_device):
+    """Zero rays and an empty tree are valid inputs."""
+    import torch
+    tree = Octree.new(16, 2)
+    d = DeviceTree(tree)
+    bufs = d.cast(torch.zeros((0, 6), dtype=torch.float32, device=gpu_device))
+    assert bufs["hit"].numel() == 0
+    rays = W.incoherent_rays(16, 1000, seed=9)
+    got = _cast(tree, rays, gpu_device)
+    assert not got["hit"].any()
+    exp = _oracle(tree).cast(rays)
+    assert_parity(exp, got, "empty tree")

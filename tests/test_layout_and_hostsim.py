@@ -1,0 +1,203 @@
+"""CPU-side checks of the product path that need no GPU:
+
+* the C ABI library loads and exports every function include/svo.h declares;
+* the flattened device layout answers get() exactly like the host octree;
+* the kernel's traversal source (traverse.hpp), compiled for the host,
+  agrees bit for bit with the oracle (the GPU tests then check the gfx950
+  build of the same source);
+* invariants of the oracle's hits and misses.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import hostsim
+import oracle
+from kat_trees import fixed_kats
+from parity import compare
+from shocovox_amd import Octree, _native, workloads as W
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_c_abi_exports_every_declared_function():
+    header = (ROOT / "include" / "svo.h").read_text()
+    declared = set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(svo_\w+)\s*\(", header, re.M))
+    declared = {d for d in declared if not d.endswith("_t")}
+    lib = C.CDLL(str(_native.LIB_PATH))
+    missing = [d for d in sorted(declared) if not hasattr(lib, d)]
+    assert not missing, f"declared but not exported: {missing}"
+    assert set(_native.exported_symbols()) <= declared
+    assert len(declared) >= 30
+
+
+def test_status_codes_and_errors():
+    with pytest.raises(_native.SvoError) as e:
+        Octree.new(12, 3)
+    assert e.value.status == 2 and "brick" in str(e.value)
+
+
+def _all_positions(n):
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1)
+    return g.reshape(-1, 3).astype(np.uint32)
+
+
+def test_flat_layout_roundtrip_plain():
+    tree = W.build_scene(W.PLAIN, seed=1)
+    flat = tree.flatten()
+    o = oracle.OracleTree(tree.export_model())
+    xyz = _all_positions(128)[::7]
+    host = tree.get_many(xyz)
+    ora = o.get_many(xyz)
+    # oracle get() returns the palette index value; host get() the entry
+    assert ((ora == 0xFFFFFFFF) == (host[:, 0] == 0)).all()
+    for p in xyz[::997]:
+        assert flat.get(tuple(int(v) for v in p)) == tree.get(tuple(int(v) for v in p))
+
+
+def test_flat_layout_structure_plain():
+    tree = W.build_scene(W.PLAIN, seed=1)
+    a = tree.flatten().arrays()
+    nodes = a["nodes"]
+    kinds = nodes["meta"] & 3
+    assert kinds[0] in (1, 2, 3)  # the root record comes first
+    internal = nodes[kinds == 1]
+    kids = internal["child"][internal["child"] != 0xFFFFFFFF]
+    assert (kids < len(nodes)).all() and len(np.unique(kids)) == len(kids)
+    # breadth-first numbering: children come after their parent
+    for i in np.nonzero(kinds == 1)[0][:50]:
+        c = nodes["child"][i]
+        assert (c[c != 0xFFFFFFFF] > i).all()
+    # occupancy sidecar equals the emptiness of the palette values
+    n3 = a["brick_dim"] ** 3
+    pix = a["brick_pix"].reshape(-1, n3)
+    occ = a["brick_occ"].reshape(len(pix), -1)
+    bits = np.unpackbits(occ.view(np.uint8), bitorder="little").reshape(len(pix), -1)[:, :n3]
+    assert np.array_equal(bits.astype(bool), pix != 0xFFFFFFFF)
+
+
+def test_flat_layout_roundtrip_kat_trees():
+    for name, (builder, _, _) in fixed_kats().items():
+        if name == "edge_case_brick_boundary_error":
+            continue
+        tree = builder()
+        flat = tree.flatten()
+        n = tree.get_size()
+        step = max(1, n // 16)
+        for x in range(0, n, step):
+            for y in range(0, n, step):
+                for z in range(0, n, step):
+                    assert flat.get((x, y, z)) == tree.get((x, y, z)), (name, x, y, z)
+
+
+def test_hostsim_equals_oracle_plain_all_rays():
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    # the kernel's ray generation compiled for the host equals the oracle's
+    assert np.array_equal(hostsim.viewport_rays(vp, cfg.width, cfg.height).view(np.uint32),
+                          rays.view(np.uint32))
+    exp = oracle.OracleTree(tree.export_model()).cast(rays)
+    got = hostsim.cast(tree.flatten().arrays(), rays)
+    res = compare(exp, {k: got[k] for k in got.dtype.names})
+    assert all(v == 0 for k, v in res.items() if k != "rays"), res
+    assert (exp["flags"] == 0).all()
+
+
+def test_hostsim_equals_oracle_kat_trees():
+    for name, (builder, r, _) in fixed_kats().items():
+        tree = builder()
+        exp = oracle.OracleTree(tree.export_model()).cast(r[None, :])
+        got = hostsim.cast(tree.flatten().arrays(), r[None, :])
+        res = compare(exp, {k: got[k] for k in got.dtype.names})
+        assert all(v == 0 for k, v in res.items() if k != "rays"), (name, res)
+
+
+def test_hostsim_equals_oracle_incoherent_small_trees():
+    rng = np.random.default_rng(21)
+    for size, dim in [(8, 1), (16, 2), (32, 4), (64, 8)]:
+        t = Octree.new(size, dim)
+        xyz = rng.integers(0, size, size=(size * size, 3)).astype(np.uint32)
+        ents = np.zeros((len(xyz), 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = (rng.integers(1, 8, size=len(xyz)).astype(np.uint32) << 24) | 0xFF
+        t.insert_many(xyz, ents)
+        rays = W.incoherent_rays(size, 4000, seed=size)
+        # plus rays from outside the bounds aimed at the centre
+        outside = rng.uniform(-size, 2 * size, size=(2000, 3)).astype(np.float32)
+        d = (np.float32(size / 2) - outside).astype(np.float32)
+        ln = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]).astype(np.float32)
+        d = (d / ln[:, None]).astype(np.float32)
+        rays = np.concatenate([rays, np.concatenate([outside, d], 1)]).astype(np.float32)
+        exp = oracle.OracleTree(t.export_model()).cast(rays)
+        got = hostsim.cast(t.flatten().arrays(), rays)
+        res = compare(exp, {k: got[k] for k in got.dtype.names})
+        assert all(v == 0 for k, v in res.items() if k != "rays"), (size, dim, res)
+
+
+def test_hostsim_equals_oracle_caves_sample():
+    cfg = W.CAVES
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width,
+                                cfg.height)[::41]
+    exp = oracle.OracleTree(tree.export_model()).cast(rays, threads=4)
+    got = hostsim.cast(tree.flatten().arrays(), rays)
+    res = compare(exp, {k: got[k] for k in got.dtype.names})
+    assert all(v == 0 for k, v in res.items() if k != "rays"), res
+
+
+def test_hit_invariants_plain():
+    """Every hit's voxel is non-empty under the reference's get() and holds the
+    hit's palette value; hit distance equals |impact - origin|."""
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    o = oracle.OracleTree(tree.export_model())
+    h = o.cast(rays)
+    hit = h["hit"] == 1
+    vox = h["voxel"][hit]
+    assert (o.get_many(vox) == h["pix"][hit]).all()
+    assert (h["flags"] == 0).all()
+    d = np.linalg.norm(h["impact"][hit].astype(np.float64) - rays[hit, :3], axis=1)
+    assert np.allclose(d, h["distance"][hit], atol=1e-3)
+
+
+def test_miss_invariant_brute_force():
+    """A miss never passes through a non-empty voxel, checked by an f64 march
+    away from cell boundaries (the reference's 0.1 restart nudge can skip
+    thin corners, so rays within 0.11 of a boundary crossing are excluded)."""
+    rng = np.random.default_rng(5)
+    t = Octree.new(32, 2)
+    xyz = rng.integers(0, 32, size=(300, 3)).astype(np.uint32)
+    ents = np.zeros((300, 3), np.uint32)
+    ents[:, 0] = 1
+    ents[:, 1] = 0xFFFFFFFF
+    t.insert_many(xyz, ents)
+    o = oracle.OracleTree(t.export_model())
+    solid = set(map(tuple, xyz.tolist()))
+    rays = W.incoherent_rays(32, 3000, seed=8)
+    h = o.cast(rays)
+    checked = bad = 0
+    for r, hh in zip(rays, h):
+        if hh["hit"]:
+            continue
+        org, d = r[:3].astype(np.float64), r[3:].astype(np.float64)
+        ts = np.arange(0.0, 60.0, 0.01)
+        pts = org[None, :] + ts[:, None] * d[None, :]
+        inside = ((pts >= 0) & (pts < 32)).all(1)
+        frac = np.abs(pts - np.round(pts))
+        near_edge = (frac < 0.11).sum(1) >= 2
+        cells = np.floor(pts[inside & ~near_edge]).astype(int)
+        checked += 1
+        if any(tuple(c) in solid for c in cells):
+            bad += 1
+    assert checked > 1000
+    assert bad <= 0.01 * checked

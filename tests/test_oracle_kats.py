@@ -1,0 +1,261 @@
+"""Pins of the oracle: the reference's own known-answer tests for the spatial
+math, the NodeStack ring, the literal LUT tables, and the ray known-answer
+tests of src/raytracing/tests.rs (on trees built by the product's insert)."""
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from kat_trees import fixed_kats, make_edge_ray_point_to, make_ray_point_to, random_fill
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def lut_literals() -> dict:
+    return json.loads((GOLDEN / "lut_literals.json").read_text())
+
+
+# --- LUTs (spatial/lut.rs) ---------------------------------------------------
+
+def test_oracle_luts_equal_reference_literals():
+    lit = lut_literals()
+    got = oracle.luts()
+    assert lit["OOB_OCTANT"] == 8
+    assert np.array_equal(np.array(lit["RAY_TO_NODE_OCCUPANCY_BITMASK_LUT"], np.uint64),
+                          got["ray_to_node_occupancy"])
+    assert np.array_equal(np.array(lit["OCTANT_STEP_RESULT_LUT"], np.uint32),
+                          got["octant_step_result"])
+    assert np.array_equal(np.array(lit["BITMAP_MASK_FOR_OCTANT_LUT"], np.uint64),
+                          got["bitmap_mask_for_octant"])
+    assert np.array_equal(np.array(lit["BITMAP_INDEX_LUT"], np.uint64), got["bitmap_index"])
+
+
+def test_product_ray_lut_equals_reference_literal():
+    from shocovox_amd import _native as N
+    out = np.zeros(512, np.uint64)
+    N.check(N.lib().svo_ray_occupancy_lut(out.ctypes.data))
+    lit = np.array(lut_literals()["RAY_TO_NODE_OCCUPANCY_BITMASK_LUT"], np.uint64).reshape(-1)
+    assert np.array_equal(out, lit)
+
+
+def test_step_lut_closed_form():
+    """OCTANT_STEP_RESULT_LUT against the closed form: step each octant axis
+    by the sign, out of {0,1} -> OOB (8)."""
+    lut = np.array(lut_literals()["OCTANT_STEP_RESULT_LUT"], np.uint32)
+    for octant in range(8):
+        ox, oz, oy = octant & 1, (octant >> 1) & 1, (octant >> 2) & 1
+        for sx in (-1, 0, 1):
+            for sy in (-1, 0, 1):
+                for sz in (-1, 0, 1):
+                    x, y, z = ox + sx, oy + sy, oz + sz
+                    exp = 8 if min(x, y, z) < 0 or max(x, y, z) > 1 else x + 2 * z + 4 * y
+                    got = (int(lut[sx + 1, sy + 1, sz + 1]) >> (4 * octant)) & 0xF
+                    assert got == exp
+
+
+def test_ray_lut_brute_force():
+    """RAY_TO_NODE_OCCUPANCY_BITMASK_LUT against brute force: the cells a
+    ray starting in a cell can reach moving monotonically along the
+    direction's signs."""
+    lut = np.array(lut_literals()["RAY_TO_NODE_OCCUPANCY_BITMASK_LUT"], np.uint64)
+    for x in range(4):
+        for y in range(4):
+            for z in range(4):
+                for dx in (-1, 1):
+                    for dy in (-1, 1):
+                        for dz in (-1, 1):
+                            d = (dx >= 0) + 2 * (dz >= 0) + 4 * (dy >= 0)
+                            mask = 0
+                            for bx in range(4):
+                                for by in range(4):
+                                    for bz in range(4):
+                                        if (bx - x) * dx >= 0 and (by - y) * dy >= 0 and \
+                                                (bz - z) * dz >= 0:
+                                            mask |= 1 << (bx + 4 * by + 16 * bz)
+                            assert int(lut[x + 4 * y + 16 * z, d]) == mask
+
+
+# --- spatial/tests.rs, spatial/math/tests.rs, spatial/raytracing/tests.rs -----
+
+def test_hash_region():  # spatial/tests.rs:22-32
+    cases = [((0, 0, 0), 0), ((6, 0, 0), 1), ((0, 0, 6), 2), ((6, 0, 6), 3), ((0, 6, 0), 4),
+             ((6, 6, 0), 5), ((0, 6, 6), 6), ((6, 6, 6), 7)]
+    for off, exp in cases:
+        assert oracle.hash_region(off, 5.0) == exp
+
+
+def test_flat_projection():  # spatial/tests.rs:42-66
+    D = 10
+    assert oracle.flat_projection(0, 0, 0, D) == 0
+    assert oracle.flat_projection(10, 0, 0, D) == D
+    assert oracle.flat_projection(0, 1, 0, D) == D
+    assert oracle.flat_projection(0, 0, 1, D) == D * D
+    assert oracle.flat_projection(0, 0, 4, D) == D * D * 4
+    assert oracle.flat_projection(3, 0, 4, D) == D * D * 4 + 3
+    assert oracle.flat_projection(3, 2, 4, D) == D * D * 4 + D * 2 + 3
+
+
+@pytest.mark.parametrize("pos,size,exp", [
+    ((0, 0, 0), 4, 0), ((0, 0, 2), 4, 32), ((3, 3, 3), 4, 63),            # :69-73
+    ((0, 0, 0), 10, 0), ((0, 0, 5), 10, 32), ((5, 5, 5), 10, 42), ((9, 9, 9), 10, 63),
+    ((0, 0, 0), 2, 0), ((1, 0, 0), 2, 2), ((0, 1, 0), 2, 8), ((1, 1, 0), 2, 10),
+    ((0, 0, 1), 2, 32), ((1, 0, 1), 2, 34), ((0, 1, 1), 2, 40), ((1, 1, 1), 2, 42),
+])
+def test_position_in_bitmap(pos, size, exp):  # spatial/tests.rs:68-93
+    assert oracle.position_in_bitmap_64bits(*pos, size) == exp
+
+
+def test_occupancy_bitmaps():  # spatial/math/tests.rs:93-186
+    m = oracle.set_occupancy((0, 0, 0), 1, 4, True, 0)
+    assert m == 0x0000000000000001
+    m = oracle.set_occupancy((3, 3, 3), 1, 4, True, m)
+    assert m == 0x8000000000000001
+    m = oracle.set_occupancy((2, 2, 2), 1, 4, True, m)
+    assert m == 0x8000040000000001
+    assert oracle.set_occupancy((0, 0, 0), 1, 1, False, 2 ** 64 - 1) == 0
+    assert oracle.set_occupancy((0, 0, 0), 1, 1, True, 0) == 2 ** 64 - 1
+    m = oracle.set_occupancy((0, 0, 0), 1, 2, True, 0)
+    assert m == 0x0000000000330033
+    assert oracle.set_occupancy((1, 1, 1), 1, 2, True, m) == 0xCC00CC0000330033
+    assert oracle.set_occupancy((0, 0, 0), 3, 4, True, 0) == 0x77707770777
+    assert oracle.set_occupancy((0, 0, 0), 2, 2, True, 0) == 0xFFFFFFFFFFFFFFFF
+    assert oracle.set_occupancy((0, 0, 0), 5, 4, True, 0) == 0xFFFFFFFFFFFFFFFF
+    assert oracle.set_occupancy((0, 0, 0), 3, 2, True, 0) == 0xFFFFFFFFFFFFFFFF
+
+
+def test_cube_intersection_kats():  # spatial/raytracing/tests.rs:89-278
+    c = ((0.0, 0.0, 0.0), 4.0)
+    assert oracle.intersect_ray(*c, (2, 5, 2), (0, -1, 0)) is not None
+    assert oracle.intersect_ray(*c, (2, -5, 2), (0, 1, 0)) is not None
+    assert oracle.intersect_ray(*c, (2, 5, 2), (0, 1, 0)) is None
+    d = np.float32(1) / np.sqrt(np.float32(3))
+    assert oracle.intersect_ray(*c, (-1, -1, -1), (d, d, d)) is not None
+    o = np.array([4, -1, 4], np.float32)
+    t = np.array([4.055, 4.055, 4.055], np.float32) - o
+    ln = np.float32(np.sqrt(np.float32(np.float32(t[0] * t[0] + t[1] * t[1]) + t[2] * t[2])))
+    assert oracle.intersect_ray(*c, o, t / ln) is None
+    v = np.array([1, 100, 1], np.float32)
+    ln = np.float32(np.sqrt(np.float32(np.float32(v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])))
+    assert oracle.intersect_ray(*c, (-1, -1, -1), v / ln) is None
+    # inwards-pointing ray on the surface: impact distance exactly 0 (a -0.0)
+    r = oracle.intersect_ray((0, 0, 0), 8.0, (8.0, 4.0, 5.0), (-0.842701, -0.24077171, -0.48154342))
+    assert r is not None and r[0] == "hit" and r[1] == 0.0
+    r = oracle.intersect_ray((0, 0, 0), 16.0, (5.0, 8.0, 5.0), (-0.48507127, -0.7276069, -0.48507127))
+    assert r == ("inside", None)
+    r = oracle.intersect_ray((0, 2, 0), 2.0, (6.0, 7.0, 6.0), (-0.6154574, -0.49236596, -0.6154574))
+    assert r is not None and r[1] > 0.0
+
+
+def test_cube_top_hit_distance():  # spatial/math/tests.rs:26-51
+    r = oracle.intersect_ray((2.0, 0.0, 0.0), 2.0, (8.965594, 10.0, -4.4292345),
+                             (-0.5082971, -0.72216684, 0.46915793))
+    assert r[0] == "hit" and abs(r[1] - 11.077772) < 0.001
+    py = np.float32(10.0) + np.float32(-0.72216684) * np.float32(r[1])
+    assert abs(py - 2.0) < 0.001
+
+
+def test_impact_normals():  # spatial/math/tests.rs:53-66
+    cases = [((1, 1, 2), (0, 0, 1)), ((1, 2, 1), (0, 1, 0)), ((2, 1, 1), (1, 0, 0)),
+             ((1, 1, 0), (0, 0, -1)), ((1, 0, 1), (0, -1, 0)), ((0, 1, 1), (-1, 0, 0))]
+    for p, n in cases:
+        assert tuple(oracle.cube_impact_normal((0, 0, 0), 2.0, p)) == n
+
+
+def test_child_bounds():  # spatial/raytracing/tests.rs:41-87
+    exp = [(0, 0, 0), (5, 0, 0), (0, 0, 5), (5, 0, 5), (0, 5, 0), (5, 5, 0), (0, 5, 5), (5, 5, 5)]
+    for octant, mn in enumerate(exp):
+        assert oracle.child_bounds((0, 0, 0), 10.0, octant) == (mn, 5.0)
+
+
+def test_node_stack():  # raytracing/tests.rs:817-911
+    POP, LAST = -1, -2
+    assert oracle.nodestack_run(4, [POP]) == [None]
+    # push 1..4 on a 3-slot ring: 1 is overwritten
+    assert oracle.nodestack_run(3, [1, 2, 3, 4, LAST, POP, LAST, POP, LAST, POP, POP]) == \
+        [4, 4, 3, 3, 2, 2, None]
+    assert oracle.nodestack_run(3, [10, 20, 30, LAST, 40, LAST]) == [30, 40]
+    assert oracle.nodestack_run(3, [5, 15, 25, POP, POP, POP, POP]) == [25, 15, 5, None]
+
+
+def test_dda_step_against_plane_intersection():
+    """The reference's independent sibling-step restatement
+    (raytracing/tests.rs:11-65) agrees with dda_step_to_next_sibling away from
+    the edge cases its ignored comparison test mentions (:90-130)."""
+    rng = np.random.default_rng(11)
+    agree = total = 0
+    for _ in range(300):
+        mn = rng.integers(0, 100, size=3).astype(np.float32)
+        size = np.float32(rng.integers(1, 1000))
+        target = mn + np.float32(0.5) * size
+        o = rng.integers(8, 16, size=3).astype(np.float32)
+        d = target - o
+        ln = np.float32(np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])))
+        d = (d / ln).astype(np.float32)
+        hit = oracle.intersect_ray(mn, size, o, d)
+        t = hit[1] if hit and hit[0] == "hit" else 0.0
+        p = (o + d * np.float32(t)).astype(np.float32)
+        _, step = oracle.dda_step(o, d, p, mn, size)
+        # plane-based reference step
+        mid = mn + size / 2
+        ref_point = mid + np.copysign(size / 2, d)
+        dist = []
+        for ax in range(3):
+            dist.append((ref_point[ax] - o[ax]) / d[ax] if d[ax] != 0 else np.float32(3.4e38))
+        m = min(dist)
+        ref = [np.copysign(1.0, d[ax]) if abs(m - dist[ax]) < 1e-5 else 0.0 for ax in range(3)]
+        total += 1
+        agree += all((step[a] != 0) == (ref[a] != 0) for a in range(3))
+    assert agree >= 0.95 * total
+
+
+# --- ray known-answer tests (raytracing/tests.rs:144-813) --------------------
+
+@pytest.mark.parametrize("name", list(fixed_kats()))
+def test_ray_kat_fixed(name):
+    builder, r, expect = fixed_kats()[name]
+    tree = builder()
+    h = oracle.OracleTree(tree.export_model()).cast(r[None, :])[0]
+    assert h["flags"] == 0
+    kind = expect[0]
+    if kind == "some":
+        assert h["hit"]
+    elif kind == "none":
+        assert not h["hit"]
+    elif kind == "data":
+        assert h["hit"] and h["albedo"] == 0 and (h["pix"] & 0xFFFF) == 0xFFFF
+    elif kind == "albedo":
+        assert h["hit"] and h["albedo"] == expect[1]
+    elif kind == "albedo_normal":
+        assert h["hit"] and h["albedo"] == expect[1] and tuple(h["normal"]) == expect[2]
+    elif kind == "albedo_normal_len":
+        assert h["hit"] and h["albedo"] == expect[1]
+        assert np.linalg.norm(h["normal"]) < expect[2]
+    elif kind == "none_or_data":
+        assert (not h["hit"]) or (h["pix"] & 0xFFFF) == 0xFFFF
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_ray_kat_random(seed):
+    """from_outside / where_dim_is_2 / from_edge / from_inside with seeded
+    randomness; every ray aimed at a filled voxel hits voxel_data!(&5)."""
+    rng = np.random.default_rng(seed)
+    cases = [
+        (*random_fill(4, 1, rng), make_ray_point_to, 0.0, False),
+        (*random_fill(4, 2, rng), make_ray_point_to, 0.0, False),
+        (*random_fill(8, 1, rng, zs=(1, 2, 3)), make_edge_ray_point_to, 0.1, False),
+        (*random_fill(16, 1, rng, zs=(1, 2, 3)), make_ray_point_to, 0.0, True),
+    ]
+    for tree, filled, maker, off, check_get in cases:
+        if not filled:
+            continue
+        o = oracle.OracleTree(tree.export_model())
+        for p in filled:
+            if check_get:
+                assert o.get(p) != 0xFFFFFFFF
+            r = maker(np.array(p, np.float32) + np.float32(off), rng)
+            h = o.cast(r[None, :])[0]
+            assert h["hit"] and (h["pix"] & 0xFFFF) == 0xFFFF and h["flags"] == 0
