@@ -86,6 +86,28 @@ def cpu_baseline(cfg, tree, vps, seconds: float) -> tuple[dict, dict]:
     return base, per_ray
 
 
+def copy_calibration(args) -> int:
+    """A plain device copy of 256 MiB per step, the reference point for
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (bytes read and written are known)."""
+    import torch
+
+    n_bytes = 256 << 20
+    src = torch.empty(n_bytes // 4, dtype=torch.int32, device="cuda:0").fill_(7)
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"mode": "copy", "bytes_read_per_step": n_bytes,
+                      "bytes_written_per_step": n_bytes, "steps": args.steps,
+                      "GBps": round(2 * n_bytes * args.steps / el / 1e9, 1)}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +117,12 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", default="auto",
+                    help="render launch variant name or index; 'auto' times each and keeps "
+                         "the fastest before the timed region")
+    ap.add_argument("--mode", default="render", choices=["render", "copy"],
+                    help="'copy' runs a plain device-to-device copy of known size (for "
+                         "calibrating rocprofv3 byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -105,7 +133,11 @@ def main() -> int:
 
     import torch
 
-    from shocovox_amd import DeviceTree, alloc_hits, collect_kernel_ms, set_timing, workloads as W
+    if args.mode == "copy":
+        return copy_calibration(args)
+
+    from shocovox_amd import (DeviceTree, alloc_hits, collect_kernel_ms, render_variants,
+                              select_render_variant, set_timing, workloads as W)
 
     cfg = W.CONFIGS[args.workload]
     t0 = time.time()
@@ -123,6 +155,26 @@ def main() -> int:
     def step():
         for vp, b in zip(vps, bufs):
             dtree.render_into(vp, cfg.width, cfg.height, b, stream)
+
+    names = render_variants()
+    tuning = {}
+    if args.variant == "auto":
+        # autotune: time every launch variant on the same frames, keep the fastest
+        set_timing(True)
+        for i, name in enumerate(names):
+            select_render_variant(i)
+            step()
+            collect_kernel_ms()
+            step()
+            step()
+            ms = collect_kernel_ms()
+            tuning[name] = round(sum(ms) / max(1, len(ms)), 4)
+        set_timing(False)
+        best = min(tuning, key=tuning.get)
+        log(f"autotune (avg ms per frame): {tuning} -> {best}")
+    else:
+        best = names[int(args.variant)] if args.variant.isdigit() else args.variant
+    select_render_variant(names.index(best))
 
     for _ in range(args.warmup):
         step()
@@ -173,7 +225,8 @@ def main() -> int:
                    "octree_size": cfg.size, "brick_dim": cfg.brick_dim, "width": cfg.width,
                    "height": cfg.height, "poses": len(vps), "rays_per_step": rays_per_step,
                    "seed": args.seed, "hit_rate": round(hits, 4),
-                   "device_bytes": dtree.device_bytes},
+                   "device_bytes": dtree.device_bytes, "launch_variant": best,
+                   "autotune_ms": tuning or None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

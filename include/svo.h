@@ -226,6 +226,16 @@ typedef struct svo_viewport {
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream);
 
+/* Launch variants of the render kernel (no counterpart in the reference,
+ * whose WGSL dispatch is fixed at 8x8 workgroups): occupancy hint, placement
+ * of the occupancy LUT (global / LDS) and per-wave tile shape. The selection
+ * is process-wide and applies to later svo_render calls; every variant
+ * computes identical results. */
+int svo_render_variant_count(void);
+const char* svo_render_variant_name(int variant);
+int svo_select_render_variant(int variant);
+int svo_current_render_variant(void);
+
 /* The RAY_TO_NODE_OCCUPANCY_BITMASK_LUT the kernels use, [pos*8 + dir]
  * (src/spatial/lut.rs:255-896), regenerated from the reference's generator
  * (lut.rs:32-85). */

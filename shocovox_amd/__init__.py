@@ -6,11 +6,12 @@ kernels for gfx950 inside libsvo.so, behind the C ABI of include/svo.h.
 from .octree import (  # noqa: F401
     COMPLEX, EMPTY, EMPTY_MARKER, INFORMATIVE, VISUAL, DeviceTree, FlatTree, Octree,
     OctreeEntry, OctreeError, Viewport, albedo, albedo_rgba, alloc_hits, collect_kernel_ms,
-    set_timing, voxel_data,
+    current_render_variant, render_variants, select_render_variant, set_timing, voxel_data,
 )
 
 __all__ = [
     "COMPLEX", "EMPTY", "EMPTY_MARKER", "INFORMATIVE", "VISUAL", "DeviceTree", "FlatTree",
     "Octree", "OctreeEntry", "OctreeError", "Viewport", "albedo", "albedo_rgba", "alloc_hits",
-    "collect_kernel_ms", "set_timing", "voxel_data",
+    "collect_kernel_ms", "current_render_variant", "render_variants", "select_render_variant",
+    "set_timing", "voxel_data",
 ]

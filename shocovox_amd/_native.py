@@ -95,6 +95,10 @@ _SIGS = {
     "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),
     "svo_render": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.POINTER(Hits), _P]),
     "svo_ray_occupancy_lut": (C.c_int, [_P]),
+    "svo_render_variant_count": (C.c_int, []),
+    "svo_render_variant_name": (C.c_char_p, [C.c_int]),
+    "svo_select_render_variant": (C.c_int, [C.c_int]),
+    "svo_current_render_variant": (C.c_int, []),
     "svo_set_timing": (C.c_int, [C.c_int]),
     "svo_collect_kernel_ms": (_U32, [C.POINTER(C.c_float), _U32]),
 }

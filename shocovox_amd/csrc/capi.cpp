@@ -348,6 +348,17 @@ int svo_ray_occupancy_lut(uint64_t* out512) {
   return SVO_OK;
 }
 
+int svo_render_variant_count(void) { return svo::render_variant_count(); }
+
+const char* svo_render_variant_name(int variant) { return svo::render_variant_name(variant); }
+
+int svo_select_render_variant(int variant) {
+  if (!svo::render_variant_select(variant)) return fail(SVO_INVALID_ARGUMENT, "no such variant");
+  return SVO_OK;
+}
+
+int svo_current_render_variant(void) { return svo::render_variant_current(); }
+
 int svo_set_timing(int enabled) {
   svo::timing_enable(enabled != 0);
   return SVO_OK;

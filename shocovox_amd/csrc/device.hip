@@ -76,17 +76,52 @@ __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __re
   store_result(t, out, i, res, o);
 }
 
-__global__ __launch_bounds__(256) void render_kernel(TreeView t, Camera cam, uint32_t w,
-                                                     uint32_t h, HitPtrs out) {
+// Primary-ray render. A workgroup is four 64-lane waves arranged 2x2; each
+// wave covers a TW x TH pixel tile (TW * TH = 64). MIN_WAVES asks the
+// compiler for at least that many waves per SIMD (it caps VGPRs to fit);
+// LDS_LUT stages the 4 KiB RAY_TO_NODE_OCCUPANCY_BITMASK_LUT in LDS.
+template <int MIN_WAVES, bool LDS_LUT, int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
+void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) {
+  static_assert(TW * TH == 64, "a wave covers 64 pixels");
+  __shared__ uint64_t s_lut[LDS_LUT ? 512 : 1];
+  const uint64_t* lut = t.ray_lut;
+  if constexpr (LDS_LUT) {
+    for (uint32_t i = threadIdx.x; i < 512; i += 256) s_lut[i] = t.ray_lut[i];
+    __syncthreads();
+    lut = s_lut;
+  }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t x = (blockIdx.x * 2u + (wave & 1u)) * 8u + (lane & 7u);
-  const uint32_t y = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane >> 3);
+  const uint32_t x = (blockIdx.x * 2u + (wave & 1u)) * TW + (lane % TW);
+  const uint32_t y = (blockIdx.y * 2u + (wave >> 1)) * TH + (lane / TW);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
-  const CastResult res = cast_ray(t, cam.origin, d);
+  const CastResult res = cast_ray(t, lut, cam.origin, d);
   store_result(t, out, y * w + x, res, cam.origin);
 }
+
+using RenderFn = void (*)(TreeView, Camera, uint32_t, uint32_t, HitPtrs);
+
+struct RenderVariant {
+  const char* name;
+  RenderFn fn;
+  uint32_t tile_w, tile_h;
+};
+
+const RenderVariant kRenderVariants[] = {
+    {"w1-glut-8x8", render_kernel<1, false, 8, 8>, 8, 8},
+    {"w5-glut-8x8", render_kernel<5, false, 8, 8>, 8, 8},
+    {"w6-glut-8x8", render_kernel<6, false, 8, 8>, 8, 8},
+    {"w8-glut-8x8", render_kernel<8, false, 8, 8>, 8, 8},
+    {"w1-slut-8x8", render_kernel<1, true, 8, 8>, 8, 8},
+    {"w6-slut-8x8", render_kernel<6, true, 8, 8>, 8, 8},
+    {"w1-glut-16x4", render_kernel<1, false, 16, 4>, 16, 4},
+    {"w1-glut-4x16", render_kernel<1, false, 4, 16>, 4, 16},
+    {"w6-slut-16x4", render_kernel<6, true, 16, 4>, 16, 4},
+};
+constexpr int kRenderVariantCount = sizeof(kRenderVariants) / sizeof(kRenderVariants[0]);
+int g_render_variant = 0;
 
 // --- timing ---------------------------------------------------------------
 
@@ -126,8 +161,10 @@ bool validate_desc(const svo_tree_desc& d, std::string* err) {
     *err = "tree has no nodes";
     return false;
   }
-  if (d.brick_dim == 0 || d.octree_size < 2 * d.brick_dim) {
-    *err = "invalid octree size / brick dimension";
+  auto pow2 = [](uint32_t v) { return v != 0 && (v & (v - 1)) == 0; };
+  if (!pow2(d.brick_dim) || !pow2(d.octree_size) || d.octree_size < 2 * d.brick_dim ||
+      d.octree_size > (1u << 24)) {
+    *err = "octree size and brick dimension must be powers of two, size >= 2 * dim, size <= 2^24";
     return false;
   }
   if (d.brick_count && (d.brick_pix == nullptr || d.brick_occ == nullptr)) {
@@ -160,6 +197,17 @@ bool validate_desc(const svo_tree_desc& d, std::string* err) {
 }  // namespace
 
 void timing_enable(bool on) { g_timing = on; }
+
+int render_variant_count() { return kRenderVariantCount; }
+const char* render_variant_name(int i) {
+  return (i >= 0 && i < kRenderVariantCount) ? kRenderVariants[i].name : nullptr;
+}
+bool render_variant_select(int i) {
+  if (i < 0 || i >= kRenderVariantCount) return false;
+  g_render_variant = i;
+  return true;
+}
+int render_variant_current() { return g_render_variant; }
 
 uint32_t timing_collect(float* out_ms, uint32_t cap) {
   std::lock_guard<std::mutex> g(g_timing_mu);
@@ -258,6 +306,7 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   t->view.dim = d.brick_dim;
   t->view.occ_words = static_cast<uint32_t>(words);
   t->view.color_count = d.color_count;
+  t->view.inv_dim = pow2_rcp(static_cast<float>(d.brick_dim));
   return SVO_OK;
 }
 
@@ -331,8 +380,9 @@ int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h
   if (w == 0 || h == 0) return SVO_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
-  const dim3 grid((w + 15) / 16, (h + 15) / 16);
-  hipLaunchKernelGGL(render_kernel, grid, dim3(256), 0, s, t.view, cam, w, h, to_ptrs(out));
+  const RenderVariant& v = kRenderVariants[g_render_variant];
+  const dim3 grid((w + 2 * v.tile_w - 1) / (2 * v.tile_w), (h + 2 * v.tile_h - 1) / (2 * v.tile_h));
+  hipLaunchKernelGGL(v.fn, grid, dim3(256), 0, s, t.view, cam, w, h, to_ptrs(out));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "render_kernel launch", err);
   timer.done();

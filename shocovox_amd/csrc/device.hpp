@@ -40,6 +40,13 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
 int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
                   const svo_hits& out, void* stream, std::string* err);
 
+// Launch variants of the render kernel (occupancy hint, LUT placement, tile
+// shape); the process-wide selection applies to later svo_render calls.
+int render_variant_count();
+const char* render_variant_name(int i);
+bool render_variant_select(int i);
+int render_variant_current();
+
 // Kernel timing with HIP events recorded on the launch stream.
 void timing_enable(bool on);
 uint32_t timing_collect(float* out_ms, uint32_t cap);

@@ -191,6 +191,32 @@ SVO_HD inline V3f dda_step(V3f dir, V3f sig, V3f* p, Cube b, V3f scale) {
   return V3f{m == dx ? sig.x : 0.0f, m == dy ? sig.y : 0.0f, m == dz ? sig.z : 0.0f};
 }
 
+// Exact reciprocal of a power of two, 0 for any other value. For s = 2^k the
+// products x * 2^-k and the quotient x / 2^k are the same real number, and
+// both operations round it once to nearest-even, so `x * pow2_rcp(s)` is
+// bit-identical to `x / s` (subnormal results included, as f32 denormals are
+// preserved). Reciprocals that would themselves be subnormal are refused.
+SVO_HD inline float pow2_rcp(float s) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, s);
+  const uint32_t e = (b >> 23) & 0xFFu;
+  if ((b & 0x807FFFFFu) != 0u || e == 0u || e >= 254u) return 0.0f;
+  return __builtin_bit_cast(float, (254u - e) << 23);
+}
+
+// x / s, through the exact reciprocal when s is a power of two.
+SVO_HD inline float div_by(float x, float s, float rcp) { return rcp != 0.0f ? x * rcp : x / s; }
+
+// a - fmodf(a, b) (spatial/math/vector.rs:66-71 as used by the POP at
+// raytracing_on_cpu.rs:452-456). For b a power of two >= 1 and 0 < a < 2^24:
+// q = floor(a * 2^-k) is exact, q * b is exact, and the reference's
+// a - (a - q b) is exactly q b, so floor(a / b) * b is the same value.
+// (+-0) - fmodf(+-0, b) is +0.
+SVO_HD inline float sub_fmod(float a, float b, float rcp_b) {
+  if (a == 0.0f) return 0.0f;
+  if (rcp_b != 0.0f && a > 0.0f && a < 16777216.0f && b >= 1.0f) return floorf(a * rcp_b) * b;
+  return a - fmodf(a, b);
+}
+
 // Palette index value codec (node.rs:354-371)
 constexpr uint32_t EMPTY_MARKER = 0xFFFFFFFFu;
 constexpr uint32_t NONE16 = 0xFFFFu;

@@ -37,6 +37,7 @@ struct TreeView {
   uint32_t dim;
   uint32_t occ_words;
   uint32_t color_count;
+  float inv_dim;  // pow2_rcp(dim): exact since the brick dimension is a power of two
 };
 
 struct CastResult {
@@ -82,13 +83,16 @@ SVO_HD inline bool probe_brick(const TreeView& t, uint32_t form, uint32_t word, 
   }
   const int dim = static_cast<int>(t.dim);
   const float fdim = static_cast<float>(t.dim);
-  const V3f pib = ((*p - bb.min) * fdim) / bb.size;
+  const float rcp_size = pow2_rcp(bb.size);
+  const V3f rel = (*p - bb.min) * fdim;
+  const V3f pib{div_by(rel.x, bb.size, rcp_size), div_by(rel.y, bb.size, rcp_size),
+                div_by(rel.z, bb.size, rcp_size)};
   auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
   int ix = clampi(f2i32(pib.x));
   int iy = clampi(f2i32(pib.y));
   int iz = clampi(f2i32(pib.z));
   int flat = ix + iy * dim + iz * dim * dim;
-  const float unit = bb.size / fdim;
+  const float unit = div_by(bb.size, fdim, t.inv_dim);
   Cube cb{bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
                        unit,
           unit};
@@ -100,11 +104,12 @@ SVO_HD inline bool probe_brick(const TreeView& t, uint32_t form, uint32_t word, 
     if ((occ[static_cast<uint32_t>(flat) >> 5] >> (static_cast<uint32_t>(flat) & 31u)) & 1u) {
       const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
       r->pix = t.brick_pix[static_cast<uint64_t>(word) * n3 + static_cast<uint32_t>(flat)];
-      const Cube hb{bb.min + (V3f{static_cast<float>(ix), static_cast<float>(iy),
-                                  static_cast<float>(iz)} *
-                              bb.size) /
-                                 fdim,
-                    bb.size / fdim};
+      const V3f scaled = V3f{static_cast<float>(ix), static_cast<float>(iy),
+                             static_cast<float>(iz)} *
+                         bb.size;
+      const Cube hb{bb.min + V3f{div_by(scaled.x, fdim, t.inv_dim), div_by(scaled.y, fdim, t.inv_dim),
+                                 div_by(scaled.z, fdim, t.inv_dim)},
+                    div_by(bb.size, fdim, t.inv_dim)};
       r->impact = *p;
       r->normal = cube_impact_normal(hb, *p);
       r->cell = hb;
@@ -120,8 +125,10 @@ SVO_HD inline bool probe_brick(const TreeView& t, uint32_t form, uint32_t word, 
   return false;
 }
 
-// get_by_ray_at_lod(ray, f32::MAX) with MIPs disabled
-SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
+// get_by_ray_at_lod(ray, f32::MAX) with MIPs disabled. `lut` is the
+// RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the caller staged it.
+template <class LutT>
+SVO_HD inline CastResult cast_ray(const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d) {
   CastResult r;
   r.hit = 0;
   r.pix = EMPTY_MARKER;
@@ -184,20 +191,23 @@ SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
           }
         }
       }
-      V3f bp = ((p - bounds.min) * 4.0f) / bounds.size;
+      const float rcp_size = pow2_rcp(bounds.size);
+      const V3f rel = (p - bounds.min) * 4.0f;
+      V3f bp{div_by(rel.x, bounds.size, rcp_size), div_by(rel.y, bounds.size, rcp_size),
+             div_by(rel.z, bounds.size, rcp_size)};
       bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
       uint32_t flat = bitmap_index(bp, &r.flags);
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
-          (occ & t.ray_lut[flat * 8u + dir_idx]) == 0ull) {
+          (occ & lut[flat * 8u + dir_idx]) == 0ull) {
         // POP (:445-475)
         head = (head + 3u) & 3u;
         count -= 1u;
         if (count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);
           const float s2 = bounds.size * 2.0f;
-          const V3f pmin =
-              bounds.min - V3f{fmodf(bounds.min.x, s2), fmodf(bounds.min.y, s2),
-                               fmodf(bounds.min.z, s2)};
+          const float rcp_s2 = pow2_rcp(s2);
+          const V3f pmin{sub_fmod(bounds.min.x, s2, rcp_s2), sub_fmod(bounds.min.y, s2, rcp_s2),
+                         sub_fmod(bounds.min.z, s2, rcp_s2)};
           const uint32_t from = hash_region(center - pmin, bounds.size);
           const V3f st = dda_step(d, sig, &p, bounds, scale);
           target = step_octant(from, st);
@@ -227,12 +237,14 @@ SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
           if (target != OOB_OCTANT) {
             tb = child_bounds_for(bounds, target);
             tck = kind == KIND_INTERNAL ? n.child[target] : INVALID_INDEX;
-            bp = bp + (st * 4.0f) / bounds.size;
+            const V3f sd = st * 4.0f;
+            bp = bp + V3f{div_by(sd.x, bounds.size, rcp_size), div_by(sd.y, bounds.size, rcp_size),
+                          div_by(sd.z, bounds.size, rcp_size)};
             flat = bitmap_index(bp, &r.flags);
           }
           if (target == OOB_OCTANT ||
               (tck != INVALID_INDEX && (occ & octant_bitmap_mask(target)) != 0ull &&
-               (t.ray_lut[flat * 8u + dir_idx] & occ) != 0ull) ||
+               (lut[flat * 8u + dir_idx] & occ) != 0ull) ||
               (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY))
             break;
         }
@@ -246,6 +258,10 @@ SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
                  : OOB_OCTANT;
   }
   return r;
+}
+
+SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
+  return cast_ray(t, t.ray_lut, o, d);
 }
 
 // Voxel coordinate of a hit: floor(impact) clamped into the hit cell.

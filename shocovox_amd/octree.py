@@ -360,6 +360,20 @@ class DeviceTree:
         return bufs
 
 
+def render_variants() -> list:
+    """Names of the render kernel's launch variants, in index order."""
+    L = N.lib()
+    return [L.svo_render_variant_name(i).decode() for i in range(L.svo_render_variant_count())]
+
+
+def select_render_variant(index: int) -> None:
+    _check(N.lib().svo_select_render_variant(index))
+
+
+def current_render_variant() -> int:
+    return int(N.lib().svo_current_render_variant())
+
+
 def set_timing(enabled: bool) -> None:
     N.lib().svo_set_timing(int(enabled))
 

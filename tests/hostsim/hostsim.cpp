@@ -29,8 +29,8 @@ struct hs_hit {
 };
 
 void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out) {
-  svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut,
-                  t->size,  t->dim,       t->occ_words, t->color_count};
+  svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut, t->size, t->dim,
+                  t->occ_words, t->color_count, svo::pow2_rcp(static_cast<float>(t->dim))};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
     const svo::V3f o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};

@@ -173,3 +173,25 @@ def test_empty_inputs(gpu_device):
     assert not got["hit"].any()
     exp = _oracle(tree).cast(rays)
     assert_parity(exp, got, "empty tree")
+
+
+def test_every_render_variant_matches_oracle(gpu_device):
+    """All launch variants of the render kernel compute the same results."""
+    import torch
+    from shocovox_amd import render_variants, select_render_variant, current_render_variant
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=4)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    exp = _oracle(tree).cast(rays)
+    d = DeviceTree(tree)
+    before = current_render_variant()
+    try:
+        for i, name in enumerate(render_variants()):
+            _say(f"render variant {name}")
+            select_render_variant(i)
+            bufs = d.render(vp, cfg.width, cfg.height)
+            torch.cuda.synchronize()
+            assert_parity(exp, gpu_to_numpy(bufs), name)
+    finally:
+        select_render_variant(before)
