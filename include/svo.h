@@ -162,9 +162,12 @@ typedef struct svo_tree_desc {
   const uint32_t* colors;    /* color_count packed albedo values */
 } svo_tree_desc;
 
-/* Flattened layout of a host octree; owned by the handle. */
+/* Flattened layout of a host octree, kept on the host; owned by the handle. */
 typedef struct svo_flat svo_flat;
 int svo_octree_flatten(const svo_octree* tree, svo_flat** out);
+/* Applies the octree's edits since the last sync to the layout in place (the
+ * host half of svo_tree_update; drains the octree's log of changed nodes). */
+int svo_flat_sync(svo_flat* flat, svo_octree* tree);
 int svo_flat_desc(const svo_flat* flat, svo_tree_desc* out);
 void svo_flat_free(svo_flat* flat);
 /* Point query on the flattened layout (layout round-trip check). */
@@ -173,12 +176,18 @@ int svo_flat_get(const svo_flat* flat, const svo_octree* tree, uint32_t x, uint3
 
 /* Upload a flattened tree to the current HIP device. */
 int svo_tree_create(const svo_tree_desc* desc, svo_tree** out);
-/* Flatten + upload in one call. */
-int svo_tree_from_octree(const svo_octree* tree, svo_tree** out);
+/* Flatten + upload in one call; the device tree keeps a host copy of its
+ * layout so that later edits can be applied in place. */
+int svo_tree_from_octree(svo_octree* tree, svo_tree** out);
 /* Bring a device tree up to date with edits made to the host octree since it
  * was created or last updated (OctreeGPUView::reload,
- * src/raytracing/bevy/mod.rs:56-60); ordered on `stream`. */
-int svo_tree_update(svo_tree* dtree, const svo_octree* tree, void* stream);
+ * src/raytracing/bevy/mod.rs:56-60): only node records and bricks that
+ * changed are uploaded (a full upload when the arrays must grow). Consumes the
+ * octree's log of changed nodes, so one host octree feeds one device tree.
+ * Ordered on `stream`; returns after the upload has completed. */
+int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream);
+/* Records and brick slots written by the last create / update. */
+int svo_tree_update_stats(const svo_tree* dtree, uint32_t* records, uint32_t* slots);
 void svo_tree_destroy(svo_tree* dtree);
 /* Device bytes held by the tree. */
 uint64_t svo_tree_device_bytes(const svo_tree* dtree);

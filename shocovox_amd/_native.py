@@ -85,11 +85,13 @@ _SIGS = {
     "svo_scene_build": (C.c_int, [_U32, _U32, _U32, C.c_uint64, C.POINTER(_P)]),
     "svo_octree_flatten": (C.c_int, [_P, C.POINTER(_P)]),
     "svo_flat_desc": (C.c_int, [_P, C.POINTER(TreeDesc)]),
+    "svo_flat_sync": (C.c_int, [_P, _P]),
     "svo_flat_free": (None, [_P]),
     "svo_flat_get": (C.c_int, [_P, _P, _U32, _U32, _U32, C.POINTER(Entry)]),
     "svo_tree_create": (C.c_int, [C.POINTER(TreeDesc), C.POINTER(_P)]),
     "svo_tree_from_octree": (C.c_int, [_P, C.POINTER(_P)]),
     "svo_tree_update": (C.c_int, [_P, _P, _P]),
+    "svo_tree_update_stats": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "svo_tree_destroy": (None, [_P]),
     "svo_tree_device_bytes": (C.c_uint64, [_P]),
     "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),

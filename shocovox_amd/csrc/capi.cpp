@@ -17,11 +17,13 @@ struct svo_octree {
 };
 
 struct svo_flat {
-  svo::FlatTree flat;
+  svo::LayoutMirror mirror;
 };
 
 struct svo_tree {
   svo::DeviceTree* dev = nullptr;
+  svo::LayoutMirror mirror;  // host copy of the layout, for in-place edits
+  bool has_mirror = false;
 };
 
 namespace {
@@ -251,14 +253,21 @@ int svo_octree_flatten(const svo_octree* tree, svo_flat** out) {
   if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   svo_flat* f = new (std::nothrow) svo_flat{};
   if (!f) return fail(SVO_INVALID_ARGUMENT, "allocation failed");
-  f->flat = svo::flatten(tree->tree);
+  f->mirror.build(tree->tree);
   *out = f;
+  return SVO_OK;
+}
+
+int svo_flat_sync(svo_flat* flat, svo_octree* tree) {
+  if (!flat || !tree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  flat->mirror.apply(tree->tree);
+  flat->mirror.mark_synced();
   return SVO_OK;
 }
 
 int svo_flat_desc(const svo_flat* flat, svo_tree_desc* out) {
   if (!flat || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
-  fill_desc(flat->flat, out);
+  fill_desc(flat->mirror.flat(), out);
   return SVO_OK;
 }
 
@@ -267,7 +276,7 @@ void svo_flat_free(svo_flat* flat) { delete flat; }
 int svo_flat_get(const svo_flat* flat, const svo_octree* tree, uint32_t x, uint32_t y, uint32_t z,
                  svo_entry* out) {
   if (!flat || !tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
-  const uint32_t pix = svo::flat_get(flat->flat, tree->tree, x, y, z);
+  const uint32_t pix = svo::flat_get(flat->mirror.flat(), tree->tree, x, y, z);
   from_entry(tree->tree.pix_get_ref(pix), out);
   return SVO_OK;
 }
@@ -283,22 +292,48 @@ int svo_tree_create(const svo_tree_desc* desc, svo_tree** out) {
   return SVO_OK;
 }
 
-int svo_tree_from_octree(const svo_octree* tree, svo_tree** out) {
+int svo_tree_from_octree(svo_octree* tree, svo_tree** out) {
   if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
-  const svo::FlatTree f = svo::flatten(tree->tree);
+  svo::LayoutMirror mirror;
+  (void)tree->tree.take_dirty();
+  mirror.build(tree->tree);
   svo_tree_desc d;
-  fill_desc(f, &d);
-  return svo_tree_create(&d, out);
+  fill_desc(mirror.flat(), &d);
+  const int st = svo_tree_create(&d, out);
+  if (st != SVO_OK) return st;
+  (*out)->mirror = std::move(mirror);
+  (*out)->mirror.mark_synced();
+  (*out)->has_mirror = true;
+  return SVO_OK;
 }
 
-int svo_tree_update(svo_tree* dtree, const svo_octree* tree, void* stream) {
+int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream) {
   if (!dtree || !tree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
-  const svo::FlatTree f = svo::flatten(tree->tree);
-  svo_tree_desc d;
-  fill_desc(f, &d);
   std::string err;
-  const int st = svo::device_tree_upload(dtree->dev, d, stream, &err);
+  if (!dtree->has_mirror) {
+    (void)tree->tree.take_dirty();
+    dtree->mirror.build(tree->tree);
+    dtree->has_mirror = true;
+  } else {
+    dtree->mirror.apply(tree->tree);
+  }
+  svo_tree_desc d;
+  fill_desc(dtree->mirror.flat(), &d);
+  int st = 1;
+  if (!dtree->mirror.full_upload())
+    st = svo::device_tree_patch(dtree->dev, d, dtree->mirror.dirty_records(),
+                                dtree->mirror.dirty_slots(), dtree->mirror.colors_synced(), stream,
+                                &err);
+  if (st == 1) st = svo::device_tree_upload(dtree->dev, d, stream, &err);
   if (st != SVO_OK) return fail(st, err);
+  dtree->mirror.mark_synced();
+  return SVO_OK;
+}
+
+int svo_tree_update_stats(const svo_tree* dtree, uint32_t* records, uint32_t* slots) {
+  if (!dtree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  if (records) *records = dtree->has_mirror ? dtree->mirror.last_records() : 0;
+  if (slots) *slots = dtree->has_mirror ? dtree->mirror.last_slots() : 0;
   return SVO_OK;
 }
 

@@ -123,6 +123,34 @@ const RenderVariant kRenderVariants[] = {
 constexpr int kRenderVariantCount = sizeof(kRenderVariants) / sizeof(kRenderVariants[0]);
 int g_render_variant = 0;
 
+// Scatter of patched records: src[i] -> dst[idx[i]] (16-byte quarters per lane)
+__global__ __launch_bounds__(256) void scatter_records_kernel(NodeRec* __restrict__ dst,
+                                                              const NodeRec* __restrict__ src,
+                                                              const uint32_t* __restrict__ idx,
+                                                              uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4u * n) return;
+  const uint32_t r = i >> 2, q = i & 3u;
+  const uint4* s4 = reinterpret_cast<const uint4*>(src + r);
+  uint4* d4 = reinterpret_cast<uint4*>(dst + idx[r]);
+  d4[q] = s4[q];
+}
+
+// Scatter of patched bricks: words [n3 + occ_words] per slot, packed in src.
+__global__ __launch_bounds__(256) void scatter_bricks_kernel(
+    uint32_t* __restrict__ pix, uint32_t* __restrict__ occ, const uint32_t* __restrict__ src,
+    const uint32_t* __restrict__ slots, uint32_t n, uint32_t n3, uint32_t words) {
+  const uint32_t per = n3 + words;
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<uint64_t>(per) * n) return;
+  const uint32_t b = static_cast<uint32_t>(i / per), w = static_cast<uint32_t>(i % per);
+  const uint64_t slot = slots[b];
+  if (w < n3)
+    pix[slot * n3 + w] = src[i];
+  else
+    occ[slot * words + (w - n3)] = src[i];
+}
+
 // --- timing ---------------------------------------------------------------
 
 struct EventPair {
@@ -345,6 +373,96 @@ int device_tree_create(const svo_tree_desc& d, DeviceTree** out, std::string* er
   return SVO_OK;
 }
 
+int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<uint32_t>& records,
+                      const std::vector<uint32_t>& slots, uint32_t colors_from, void* stream,
+                      std::string* err) {
+  if (d.node_count > t->node_capacity || (d.brick_count > t->brick_capacity) ||
+      d.color_count > t->color_capacity || d.brick_dim != t->view.dim)
+    return 1;  // caller re-uploads everything
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSetDevice(t->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
+  for (uint32_t r : records)
+    if (r >= d.node_count) {
+      *err = "patched record out of range";
+      return SVO_INVALID_ARGUMENT;
+    }
+  for (uint32_t b : slots)
+    if (b >= d.brick_count) {
+      *err = "patched brick slot out of range";
+      return SVO_INVALID_ARGUMENT;
+    }
+  const uint32_t n3 = d.brick_dim * d.brick_dim * d.brick_dim;
+  const uint32_t words = occ_words_per_brick(d.brick_dim);
+  // host staging: [records][record indices][brick words][slot indices]
+  std::vector<NodeRec> rec(records.size());
+  for (size_t i = 0; i < records.size(); ++i)
+    std::memcpy(&rec[i], &d.nodes[records[i]], sizeof(NodeRec));
+  std::vector<uint32_t> bw(static_cast<size_t>(slots.size()) * (n3 + words));
+  for (size_t i = 0; i < slots.size(); ++i) {
+    std::memcpy(&bw[i * (n3 + words)], d.brick_pix + static_cast<size_t>(slots[i]) * n3,
+                n3 * sizeof(uint32_t));
+    std::memcpy(&bw[i * (n3 + words) + n3], d.brick_occ + static_cast<size_t>(slots[i]) * words,
+                words * sizeof(uint32_t));
+  }
+  const size_t bytes = rec.size() * sizeof(NodeRec) + records.size() * 4 + bw.size() * 4 +
+                       slots.size() * 4 + 256;
+  if (bytes > t->staging_bytes) {
+    if (t->staging) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(t->staging);
+      t->staging = nullptr;
+      t->staging_bytes = 0;
+    }
+    e = hipMalloc(&t->staging, bytes + bytes / 2);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(staging)", err);
+    t->staging_bytes = bytes + bytes / 2;
+  }
+  char* base = static_cast<char*>(t->staging);
+  NodeRec* d_rec = reinterpret_cast<NodeRec*>(base);
+  uint32_t* d_ridx = reinterpret_cast<uint32_t*>(d_rec + rec.size());
+  uint32_t* d_bw = d_ridx + records.size();
+  // keep the brick words 16-byte aligned
+  d_bw = reinterpret_cast<uint32_t*>((reinterpret_cast<uintptr_t>(d_bw) + 15) & ~uintptr_t(15));
+  uint32_t* d_slots = d_bw + bw.size();
+  if (!records.empty()) {
+    e = hipMemcpyAsync(d_rec, rec.data(), rec.size() * sizeof(NodeRec), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_ridx, records.data(), records.size() * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(records)", err);
+    const uint32_t n = static_cast<uint32_t>(records.size());
+    hipLaunchKernelGGL(scatter_records_kernel, dim3((4 * n + 255) / 256), dim3(256), 0, s,
+                       t->nodes, d_rec, d_ridx, n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "scatter_records_kernel", err);
+  }
+  if (!slots.empty()) {
+    e = hipMemcpyAsync(d_bw, bw.data(), bw.size() * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(bricks)", err);
+    const uint64_t total = static_cast<uint64_t>(slots.size()) * (n3 + words);
+    hipLaunchKernelGGL(scatter_bricks_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)),
+                       dim3(256), 0, s, t->brick_pix, t->brick_occ, d_bw, d_slots,
+                       static_cast<uint32_t>(slots.size()), n3, words);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "scatter_bricks_kernel", err);
+  }
+  if (d.color_count > colors_from) {
+    e = hipMemcpyAsync(t->colors + colors_from, d.colors + colors_from,
+                       (d.color_count - colors_from) * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(colors)", err);
+  }
+  // the host staging vectors die with this call
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize", err);
+  t->node_count = d.node_count;
+  t->brick_count = d.brick_count;
+  t->color_count = d.color_count;
+  t->view.color_count = d.color_count;
+  return SVO_OK;
+}
+
 void device_tree_destroy(DeviceTree* t) {
   if (!t) return;
   if (t->device >= 0) (void)hipSetDevice(t->device);
@@ -354,6 +472,7 @@ void device_tree_destroy(DeviceTree* t) {
   if (t->brick_occ) (void)hipFree(t->brick_occ);
   if (t->colors) (void)hipFree(t->colors);
   if (t->ray_lut) (void)hipFree(t->ray_lut);
+  if (t->staging) (void)hipFree(t->staging);
   delete t;
 }
 

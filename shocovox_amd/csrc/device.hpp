@@ -3,6 +3,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/svo.h"
 #include "layout.hpp"
@@ -24,6 +25,8 @@ struct DeviceTree {
   uint32_t* brick_occ = nullptr;
   uint32_t* colors = nullptr;
   uint64_t* ray_lut = nullptr;
+  void* staging = nullptr;  // device staging of incremental patches
+  uint64_t staging_bytes = 0;
   uint64_t bytes = 0;
   TreeView view{};
 };
@@ -34,6 +37,13 @@ int device_tree_create(const svo_tree_desc& desc, DeviceTree** out, std::string*
 // needed; ordered on `stream`.
 int device_tree_upload(DeviceTree* t, const svo_tree_desc& desc, void* stream, std::string* err);
 void device_tree_destroy(DeviceTree* t);
+// Writes the given records and brick slots of a layout into an existing tree
+// whose capacities already cover `desc` (the layout's full description);
+// returns SVO_OUT_OF_DEVICE_MEMORY-free status or asks for a full upload by
+// returning 1 when the capacities are too small.
+int device_tree_patch(DeviceTree* t, const svo_tree_desc& desc,
+                      const std::vector<uint32_t>& records, const std::vector<uint32_t>& slots,
+                      uint32_t colors_from, void* stream, std::string* err);
 
 int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hits& out,
                 void* stream, std::string* err);

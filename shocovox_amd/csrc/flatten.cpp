@@ -2,6 +2,8 @@
 #include "flatten.hpp"
 
 #include <cmath>
+#include <algorithm>
+#include <cstring>
 #include <unordered_map>
 
 namespace svo {
@@ -178,6 +180,187 @@ void generate_ray_occupancy_lut(uint64_t out[64 * 8]) {
               out[pos * 8 + dir] = mask;
             }
       }
+}
+
+}  // namespace svo
+
+namespace svo {
+
+namespace {
+constexpr uint32_t NO_RECORD = 0xFFFFFFFFu;
+}
+
+void LayoutMirror::build(const Octree& tree) {
+  flat_ = flatten(tree);
+  key_to_rec_.assign(tree.nodes().len(), NO_RECORD);
+  // recover key -> record from the breadth-first order flatten() used
+  std::vector<uint32_t> order{Octree::ROOT_NODE_KEY};
+  key_to_rec_[Octree::ROOT_NODE_KEY] = 0;
+  for (size_t i = 0; i < order.size(); ++i) {
+    const uint32_t key = order[i];
+    if (tree.nodes().get(key).kind != NodeKind::Internal) continue;
+    for (uint32_t o = 0; o < 8; ++o) {
+      const uint32_t c = tree.node_children()[key].child(o);
+      if (!tree.nodes().key_is_valid(c) || key_to_rec_[c] != NO_RECORD) continue;
+      key_to_rec_[c] = static_cast<uint32_t>(order.size());
+      order.push_back(c);
+    }
+  }
+  free_recs_.clear();
+  free_slots_.clear();
+  dirty_recs_.clear();
+  dirty_slots_.clear();
+  rec_dirty_flag_.assign(flat_.nodes.size(), 0);
+  slot_dirty_flag_.assign(flat_.brick_count(), 0);
+  full_ = true;
+  colors_synced_ = 0;
+}
+
+uint32_t LayoutMirror::record_for(uint32_t key) {
+  if (key >= key_to_rec_.size()) key_to_rec_.resize(static_cast<size_t>(key) + 1, NO_RECORD);
+  if (key_to_rec_[key] != NO_RECORD) return key_to_rec_[key];
+  uint32_t r;
+  if (!free_recs_.empty()) {
+    r = free_recs_.back();
+    free_recs_.pop_back();
+  } else {
+    r = static_cast<uint32_t>(flat_.nodes.size());
+    NodeRec blank{};
+    blank.meta = KIND_NOTHING;
+    for (auto& c : blank.child) c = INVALID_INDEX;
+    flat_.nodes.push_back(blank);
+    rec_dirty_flag_.push_back(0);
+  }
+  key_to_rec_[key] = r;
+  return r;
+}
+
+void LayoutMirror::release_slot(uint32_t slot) { free_slots_.push_back(slot); }
+
+void LayoutMirror::release_key(uint32_t key) {
+  if (key >= key_to_rec_.size() || key_to_rec_[key] == NO_RECORD) return;
+  const uint32_t r = key_to_rec_[key];
+  NodeRec& rec = flat_.nodes[r];
+  const uint32_t kind = node_kind(rec.meta);
+  if (kind == KIND_LEAF || kind == KIND_UNIFORM_LEAF)
+    for (uint32_t o = 0; o < 8; ++o)
+      if (brick_form(rec.meta, o) == FORM_PARTED) release_slot(rec.child[o]);
+  rec.meta = KIND_NOTHING;
+  rec.occ = 0;
+  key_to_rec_[key] = NO_RECORD;
+  if (r != 0) free_recs_.push_back(r);
+}
+
+uint32_t LayoutMirror::write_brick(const Octree& tree, const Brick& b, uint32_t old_slot) {
+  const uint32_t dim = flat_.brick_dim;
+  const size_t n3 = static_cast<size_t>(dim) * dim * dim;
+  const uint32_t words = occ_words_per_brick(dim);
+  uint32_t slot = old_slot;
+  bool fresh = false;
+  if (slot == NO_RECORD) {
+    fresh = true;
+    if (!free_slots_.empty()) {
+      slot = free_slots_.back();
+      free_slots_.pop_back();
+    } else {
+      slot = flat_.brick_count();
+      flat_.brick_pix.resize(flat_.brick_pix.size() + n3, EMPTY_MARKER);
+      flat_.brick_occ.resize(flat_.brick_occ.size() + words, 0u);
+      slot_dirty_flag_.push_back(0);
+    }
+  }
+  uint32_t* pix = flat_.brick_pix.data() + static_cast<size_t>(slot) * n3;
+  if (!fresh && std::equal(b.parted.begin(), b.parted.end(), pix)) return slot;
+  std::copy(b.parted.begin(), b.parted.end(), pix);
+  uint32_t* occ = flat_.brick_occ.data() + static_cast<size_t>(slot) * words;
+  std::fill(occ, occ + words, 0u);
+  for (size_t i = 0; i < n3; ++i)
+    if (!tree.pix_points_to_empty(pix[i])) occ[i / 32] |= 1u << (i % 32);
+  if (!slot_dirty_flag_[slot]) {
+    slot_dirty_flag_[slot] = 1;
+    dirty_slots_.push_back(slot);
+  }
+  return slot;
+}
+
+void LayoutMirror::write_record(const Octree& tree, uint32_t key, std::vector<uint32_t>* pending) {
+  const uint32_t r = record_for(key);
+  const NodeContent& n = tree.nodes().get(key);
+  const NodeRec before = flat_.nodes[r];
+  NodeRec rec{};
+  for (auto& c : rec.child) c = INVALID_INDEX;
+  rec.occ = tree.stored_occupied_bits(key);
+  const auto& kids = tree.node_children();
+  // slots the record held before, per octant
+  uint32_t old_slots[8];
+  const uint32_t old_kind = node_kind(before.meta);
+  for (uint32_t o = 0; o < 8; ++o)
+    old_slots[o] = ((old_kind == KIND_LEAF || old_kind == KIND_UNIFORM_LEAF) &&
+                    brick_form(before.meta, o) == FORM_PARTED)
+                       ? before.child[o]
+                       : NO_RECORD;
+  auto encode = [&](const Brick& b, uint32_t o) {
+    if (b.kind == BrickKind::Solid) {
+      rec.meta |= FORM_SOLID << (8u + 2u * o);
+      rec.child[o] = b.solid;
+    } else if (b.kind == BrickKind::Parted) {
+      rec.meta |= FORM_PARTED << (8u + 2u * o);
+      rec.child[o] = write_brick(tree, b, old_slots[o]);
+      old_slots[o] = NO_RECORD;
+    }
+  };
+  switch (n.kind) {
+    case NodeKind::Nothing:
+      rec.meta = KIND_NOTHING;
+      break;
+    case NodeKind::Internal:
+      rec.meta = KIND_INTERNAL;
+      for (uint32_t o = 0; o < 8; ++o) {
+        const uint32_t c = key < kids.size() ? kids[key].child(o) : EMPTY_MARKER;
+        if (!tree.nodes().key_is_valid(c)) continue;
+        const bool had = c < key_to_rec_.size() && key_to_rec_[c] != NO_RECORD;
+        rec.child[o] = record_for(c);
+        if (!had) pending->push_back(c);
+      }
+      break;
+    case NodeKind::Leaf:
+      rec.meta = KIND_LEAF;
+      for (uint32_t o = 0; o < 8; ++o) encode(n.bricks[o], o);
+      break;
+    case NodeKind::UniformLeaf:
+      rec.meta = KIND_UNIFORM_LEAF;
+      encode(n.bricks[0], 0);
+      break;
+  }
+  for (uint32_t o = 0; o < 8; ++o)
+    if (old_slots[o] != NO_RECORD) release_slot(old_slots[o]);
+  if (std::memcmp(&rec, &before, sizeof(NodeRec)) != 0 || r >= rec_dirty_flag_.size()) {
+    flat_.nodes[r] = rec;
+    if (!rec_dirty_flag_[r]) {
+      rec_dirty_flag_[r] = 1;
+      dirty_recs_.push_back(r);
+    }
+  }
+}
+
+void LayoutMirror::apply(Octree& tree) {
+  std::vector<uint32_t> keys = tree.take_dirty();
+  flat_.colors = tree.color_palette();
+  // released nodes first, so their records and slots can be reused
+  for (uint32_t k : keys)
+    if (!tree.nodes().key_is_valid(k)) release_key(k);
+  std::vector<uint32_t> pending;
+  for (uint32_t k : keys)
+    if (tree.nodes().key_is_valid(k)) pending.push_back(k);
+  std::vector<uint8_t> done(tree.nodes().len(), 0);
+  for (size_t i = 0; i < pending.size(); ++i) {
+    const uint32_t k = pending[i];
+    if (done[k]) continue;
+    done[k] = 1;
+    write_record(tree, k, &pending);
+  }
+  for (uint32_t r : dirty_recs_) rec_dirty_flag_[r] = 0;
+  for (uint32_t s : dirty_slots_) slot_dirty_flag_[s] = 0;
 }
 
 }  // namespace svo

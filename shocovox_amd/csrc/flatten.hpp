@@ -41,3 +41,55 @@ uint32_t flat_get(const FlatTree& f, const Octree& palette_owner, uint32_t x, ui
 void generate_ray_occupancy_lut(uint64_t out[64 * 8]);
 
 }  // namespace svo
+
+namespace svo {
+
+// A host copy of a tree's device layout that follows edits of the host
+// octree: records of dirty pool keys are rewritten in place, new nodes and
+// bricks get new records / slots (or reuse released ones), and the indices
+// that changed are listed so that only those are uploaded.
+class LayoutMirror {
+ public:
+  // Full build (breadth-first, as flatten()). The tree's log of changed
+  // nodes is left alone; the caller drains it when this mirror is its reader.
+  void build(const Octree& tree);
+  // Applies the tree's pending dirty keys.
+  void apply(Octree& tree);
+
+  const FlatTree& flat() const { return flat_; }
+  // Record indices / brick slots written by the last build or apply.
+  const std::vector<uint32_t>& dirty_records() const { return dirty_recs_; }
+  const std::vector<uint32_t>& dirty_slots() const { return dirty_slots_; }
+  bool full_upload() const { return full_; }
+  uint32_t colors_synced() const { return colors_synced_; }
+  // Sizes of the last patch (records, brick slots) that mark_synced() retired.
+  uint32_t last_records() const { return last_recs_; }
+  uint32_t last_slots() const { return last_slots_; }
+  void mark_synced() {
+    last_recs_ = full_ ? static_cast<uint32_t>(flat_.nodes.size())
+                       : static_cast<uint32_t>(dirty_recs_.size());
+    last_slots_ = full_ ? flat_.brick_count() : static_cast<uint32_t>(dirty_slots_.size());
+    dirty_recs_.clear();
+    dirty_slots_.clear();
+    full_ = false;
+    colors_synced_ = static_cast<uint32_t>(flat_.colors.size());
+  }
+
+ private:
+  uint32_t record_for(uint32_t key);
+  void write_record(const Octree& tree, uint32_t key, std::vector<uint32_t>* pending);
+  uint32_t write_brick(const Octree& tree, const Brick& b, uint32_t old_slot);
+  void release_slot(uint32_t slot);
+  void release_key(uint32_t key);
+
+  FlatTree flat_;
+  std::vector<uint32_t> key_to_rec_;
+  std::vector<uint32_t> free_recs_, free_slots_;
+  std::vector<uint32_t> dirty_recs_, dirty_slots_;
+  std::vector<uint8_t> rec_dirty_flag_, slot_dirty_flag_;
+  bool full_ = true;
+  uint32_t colors_synced_ = 0;
+  uint32_t last_recs_ = 0, last_slots_ = 0;
+};
+
+}  // namespace svo

@@ -181,6 +181,7 @@ uint32_t NodePool::push(NodeContent item) {
   }
   if (is_next_available()) first_available_ += 1;
   items_[key] = std::move(item);
+  touch(static_cast<uint32_t>(key));
   return static_cast<uint32_t>(key);
 }
 
@@ -188,10 +189,13 @@ bool NodePool::free(uint32_t key) {
   if (!key_is_valid(key)) return false;
   reserved_[key] = 0;
   first_available_ = std::min<size_t>(first_available_, key);
+  touch(key);
   return true;
 }
 
 void NodePool::swap(uint32_t a, uint32_t b) {
+  touch(a);
+  touch(b);
   std::swap(items_[a], items_[b]);
   std::swap(reserved_[a], reserved_[b]);
 }
@@ -371,13 +375,13 @@ void Octree::store_occupied_bits(uint32_t key, uint64_t bits) {
       n.bits = bits;
       break;
     case NodeKind::Nothing:
-      node_children_[key] = NodeChildren::bitmap_of(bits);
+      kids(key) = NodeChildren::bitmap_of(bits);
       break;
     case NodeKind::Leaf:
     case NodeKind::UniformLeaf:
       // The reference panics for Children here; the edit paths never reach it.
       if (node_children_[key].kind != ChildKind::Children)
-        node_children_[key] = NodeChildren::bitmap_of(bits);
+        kids(key) = NodeChildren::bitmap_of(bits);
       break;
   }
 }
@@ -689,7 +693,7 @@ void Octree::subdivide_leaf_to_nodes(uint32_t node_key, uint32_t target_octant) 
   NodeChildren nc;
   nc.kind = ChildKind::Children;
   for (int o = 0; o < 8; ++o) nc.c[o] = new_children[o];
-  node_children_[node_key] = nc;
+  kids(node_key) = nc;
 }
 
 Brick Octree::try_brick_from_node(uint32_t key) const {
@@ -708,7 +712,7 @@ void Octree::deallocate_children_of(uint32_t node) {
   for (uint32_t c : to_free) {
     deallocate_children_of(c);
     nodes_.free(c);
-    node_children_[c] = NodeChildren{};
+    kids(c) = NodeChildren{};
   }
 }
 
@@ -828,7 +832,7 @@ uint64_t Octree::leaf_update(bool overwrite_if_empty, uint32_t node_key, Cube no
     }
     case NodeKind::Internal: {
       const uint64_t ocbits = node.bits;
-      node_children_[node_key] = NodeChildren::bitmap_of(ocbits);
+      kids(node_key) = NodeChildren::bitmap_of(ocbits);
       // child(i) of an OccupancyBitmap is always invalid, so every brick is Empty
       NodeContent leaf;
       leaf.kind = NodeKind::Leaf;
@@ -868,7 +872,7 @@ bool Octree::simplify(uint32_t key) {
       if (b.kind == BrickKind::Solid) {
         if (pix_points_to_empty(b.solid)) {
           n = NodeContent::nothing();
-          node_children_[key] = NodeChildren{};
+          kids(key) = NodeChildren{};
           return true;
         }
         return false;
@@ -952,23 +956,23 @@ bool Octree::simplify(uint32_t key) {
         return true;
       }
       if (node_children_[key].kind != ChildKind::Children) return false;
-      uint32_t kids[8];
-      for (int o = 0; o < 8; ++o) kids[o] = node_children_[key].c[o];
-      simplify(kids[0]);
-      if (!nodes_.key_is_valid(kids[0])) {
-        for (int o = 1; o < 8; ++o) simplify(kids[o]);
+      uint32_t child_keys[8];
+      for (int o = 0; o < 8; ++o) child_keys[o] = node_children_[key].c[o];
+      simplify(child_keys[0]);
+      if (!nodes_.key_is_valid(child_keys[0])) {
+        for (int o = 1; o < 8; ++o) simplify(child_keys[o]);
         return false;
       }
       for (int o = 1; o < 8; ++o) {
-        simplify(kids[o]);
-        if (!nodes_.key_is_valid(kids[o]) ||
-            !content_compare(nodes_.get(kids[0]), nodes_.get(kids[o])))
+        simplify(child_keys[o]);
+        if (!nodes_.key_is_valid(child_keys[o]) ||
+            !content_compare(nodes_.get(child_keys[0]), nodes_.get(child_keys[o])))
           return false;
       }
-      nodes_.swap(key, kids[0]);
-      const NodeChildren promoted = node_children_[kids[0]];
+      nodes_.swap(key, child_keys[0]);
+      const NodeChildren promoted = node_children_[child_keys[0]];
       deallocate_children_of(key);
-      node_children_[key] = promoted;
+      kids(key) = promoted;
       return true;
     }
   }
@@ -1010,12 +1014,12 @@ Status Octree::insert_at_lod_internal(bool overwrite_if_empty, uint32_t x, uint3
       if (nodes_.key_is_valid(tck)) {
         deallocate_children_of(tck);
         nodes_.get(tck) = NodeContent::uniform(Brick::make_solid(tc));
-        node_children_[tck] = NodeChildren::bitmap_of(~0ull);
+        kids(tck) = NodeChildren::bitmap_of(~0ull);
       } else {
         const uint32_t nk = push_node(NodeContent::uniform(Brick::make_solid(tc)));
         if (node_children_.size() < static_cast<size_t>(nk) + 1) node_children_.resize(nk + 1);
-        node_children_[ck].child_mut(oct) = nk;
-        node_children_[nk] = NodeChildren::bitmap_of(~0ull);
+        kids(ck).child_mut(oct) = nk;
+        kids(nk) = NodeChildren::bitmap_of(~0ull);
       }
       actual_update_size = f2usize(tb.size);
       break;
@@ -1053,7 +1057,7 @@ Status Octree::insert_at_lod_internal(bool overwrite_if_empty, uint32_t x, uint3
         if (nodes_.get(ck).kind == NodeKind::Nothing) nodes_.get(ck) = NodeContent::internal(0);
         const uint32_t nk = push_node(NodeContent::nothing());
         if (node_children_.size() < nodes_.len()) node_children_.resize(nodes_.len());
-        node_children_[ck].child_mut(oct) = nk;
+        kids(ck).child_mut(oct) = nk;
         stack.push_back({nk, tb});
       }
     } else {
@@ -1115,7 +1119,7 @@ Status Octree::clear_at_lod(uint32_t x, uint32_t y, uint32_t z, uint32_t clear_s
         nodes_.key_is_valid(tck)) {
       deallocate_children_of(tck);
       nodes_.get(tck) = NodeContent::nothing();
-      node_children_[tck] = NodeChildren{};
+      kids(tck) = NodeChildren{};
       actual_update_size = f2usize(tb.size);
       stack.push_back({node_children_[ck].child(oct), tb});
       break;
@@ -1168,7 +1172,7 @@ Status Octree::clear_at_lod(uint32_t x, uint32_t y, uint32_t z, uint32_t clear_s
       const uint32_t co =
           hash_region((removed.bounds.min - nb.min) + v3u(removed.bounds.size / 2.0f),
                       nb.size / 2.0f);
-      node_children_[key].clear(co);
+      kids(key).clear(co);
       nodes_.free(removed.key);
       have_removed = false;
     }
@@ -1193,13 +1197,13 @@ Status Octree::clear_at_lod(uint32_t x, uint32_t y, uint32_t z, uint32_t clear_s
       nodes_.get(key) = NodeContent::internal(bits);
     } else {
       deallocate_children_of(key);
-      node_children_[key] = NodeChildren{};
+      kids(key) = NodeChildren{};
       have_removed = true;
       removed = *it;
       nodes_.get(key) = NodeContent::nothing();
     }
     if (bits == 0)
-      node_children_[key] = NodeChildren{};
+      kids(key) = NodeChildren{};
     else
       store_occupied_bits(key, bits);
     if (simplifyable) simplifyable = simplify(key);

@@ -115,12 +115,17 @@ enum class Status : int {
 };
 
 // ObjectPool<NodeContent> (object_pool.rs:87-241)
+// Every mutable access records the key as dirty (conservatively: a mutable
+// reference counts as a write), so a device copy can be patched in place.
 class NodePool {
  public:
   uint32_t push(NodeContent item);
   bool free(uint32_t key);
   bool key_is_valid(uint64_t key) const { return key < items_.size() && reserved_[key]; }
-  NodeContent& get(uint32_t key) { return items_[key]; }
+  NodeContent& get(uint32_t key) {
+    touch(key);
+    return items_[key];
+  }
   const NodeContent& get(uint32_t key) const { return items_[key]; }
   void swap(uint32_t a, uint32_t b);
   uint32_t len() const { return static_cast<uint32_t>(items_.size()); }
@@ -129,6 +134,20 @@ class NodePool {
     items_.reserve(n);
     reserved_.reserve(n);
   }
+  void touch(uint32_t key) {
+    if (key >= dirty_flag_.size()) dirty_flag_.resize(static_cast<size_t>(key) + 1, 0);
+    if (!dirty_flag_[key]) {
+      dirty_flag_[key] = 1;
+      dirty_.push_back(key);
+    }
+  }
+  // Keys written since the last call, in first-touch order.
+  std::vector<uint32_t> take_dirty() {
+    for (uint32_t k : dirty_) dirty_flag_[k] = 0;
+    std::vector<uint32_t> out;
+    out.swap(dirty_);
+    return out;
+  }
 
  private:
   bool is_next_available() const;
@@ -136,6 +155,8 @@ class NodePool {
   std::vector<NodeContent> items_;
   std::vector<uint8_t> reserved_;
   size_t first_available_ = 0;
+  std::vector<uint8_t> dirty_flag_;
+  std::vector<uint32_t> dirty_;
 };
 
 class Octree {
@@ -173,6 +194,8 @@ class Octree {
   NodePool& nodes_mut() { return nodes_; }
   const std::vector<NodeChildren>& node_children() const { return node_children_; }
   std::vector<NodeChildren>& node_children_mut() { return node_children_; }
+  // Pool keys whose node or children changed since the last call.
+  std::vector<uint32_t> take_dirty() { return nodes_.take_dirty(); }
   const std::vector<uint32_t>& color_palette() const { return color_palette_; }
   const std::vector<uint32_t>& data_palette() const { return data_palette_; }
   uint64_t stored_occupied_bits(uint32_t key) const;
@@ -207,6 +230,11 @@ class Octree {
   bool should_bitmap_be_empty_at_octants(uint32_t node_key, uint32_t target_octant,
                                          uint32_t target_octant_for_child) const;
   bool node_empty_at(uint32_t node_key, uint32_t octant) const;
+
+  NodeChildren& kids(uint32_t key) {
+    nodes_.touch(key);
+    return node_children_[key];
+  }
 
   uint32_t brick_dim_ = 1;
   uint32_t octree_size_ = 2;

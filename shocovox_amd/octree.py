@@ -246,6 +246,10 @@ class FlatTree:
             "colors": view(d.colors, d.color_count, np.uint32),
         }
 
+    def sync(self, octree: Octree) -> None:
+        """Applies the octree's edits since the last sync in place."""
+        _check(N.lib().svo_flat_sync(self._h, octree.handle))
+
     def get(self, pos) -> OctreeEntry:
         e = N.Entry()
         _check(N.lib().svo_flat_get(self._h, self._owner.handle, pos[0], pos[1], pos[2],
@@ -331,9 +335,13 @@ class DeviceTree:
     def device_bytes(self) -> int:
         return int(N.lib().svo_tree_device_bytes(self._h))
 
-    def update(self, octree: Octree, stream=None) -> None:
-        """Re-syncs the device copy with the host octree (OctreeGPUView::reload)."""
+    def update(self, octree: Octree, stream=None) -> tuple:
+        """Re-syncs the device copy with the host octree (OctreeGPUView::reload):
+        uploads only the records and bricks that changed; returns their counts."""
         _check(N.lib().svo_tree_update(self._h, octree.handle, _stream_ptr(stream)))
+        r, s = C.c_uint32(), C.c_uint32()
+        _check(N.lib().svo_tree_update_stats(self._h, C.byref(r), C.byref(s)))
+        return int(r.value), int(s.value)
 
     def cast_into(self, rays, n: int, bufs: dict, stream=None) -> None:
         hs = _hits_struct(bufs)

@@ -20,7 +20,7 @@ import hostsim
 import oracle
 from kat_trees import fixed_kats
 from parity import compare
-from shocovox_amd import Octree, _native, workloads as W
+from shocovox_amd import Octree, OctreeEntry, _native, workloads as W
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -201,3 +201,43 @@ def test_miss_invariant_brute_force():
             bad += 1
     assert checked > 1000
     assert bad <= 0.01 * checked
+
+
+def _random_edits(tree, rng, n_insert, n_clear, size, colors=6):
+    xyz = rng.integers(0, size, size=(n_insert, 3)).astype(np.uint32)
+    ents = np.zeros((n_insert, 3), np.uint32)
+    ents[:, 0] = 1
+    ents[:, 1] = (rng.integers(1, colors, size=n_insert).astype(np.uint32) << 16) | 0xFF
+    tree.insert_many(xyz, ents)
+    tree.clear_many(rng.integers(0, size, size=(n_clear, 3)).astype(np.uint32))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_incremental_layout_matches_fresh_tree(seed):
+    """Edits applied in place to the layout (svo_flat_sync) answer casts and
+    point queries exactly like the edited tree (oracle) after every round."""
+    rng = np.random.default_rng(seed)
+    size, dim = [(32, 2), (64, 4), (128, 8), (64, 1)][seed]
+    tree = Octree.new(size, dim)
+    _random_edits(tree, rng, 2000, 0, size)
+    flat = tree.flatten()
+    flat.sync(tree)  # drains the build's log
+    rays = W.incoherent_rays(size, 3000, seed=seed + 40)
+    for _ in range(6):
+        _random_edits(tree, rng, int(rng.integers(1, 400)), int(rng.integers(0, 400)), size)
+        if rng.random() < 0.3:  # whole-node edits as well
+            p = tuple(int(v) for v in rng.integers(0, size, size=3))
+            s = int(rng.choice([2, 4, 8]))
+            if rng.random() < 0.5:
+                tree.insert_at_lod(p, s, OctreeEntry.visual(0x77AA33FF))
+            else:
+                tree.clear_at_lod(p, s)
+        flat.sync(tree)
+        a = flat.arrays()
+        exp = oracle.OracleTree(tree.export_model()).cast(rays)
+        got = hostsim.cast(a, rays)
+        res = compare(exp, {k: got[k] for k in got.dtype.names})
+        assert all(v == 0 for k, v in res.items() if k != "rays"), res
+        for p in rng.integers(0, size, size=(300, 3)):
+            p = tuple(int(v) for v in p)
+            assert flat.get(p) == tree.get(p), p
