@@ -203,9 +203,13 @@ def main() -> int:
         b_ray = sum(EVENT_BYTES[k] * v for k, v in per_ray.items()) + OUT_BYTES_PER_RAY
         achieved = b_ray * rays_per_frame / (avg_launch_ms * 1e-3) / 1e9
         traffic = None
-        tf = ROOT / "profiles" / f"traffic_{cfg.name}.json"
+        # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+        # workload, scaled by the same counters' reading of a copy of known size
+        tf = ROOT / "profiles" / f"render_kernel_{cfg.name}_{cfg.height}p_traffic.json"
         if tf.exists():
-            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+            tj = json.loads(tf.read_text())
+            if tj.get("units_per_launch") == rays_per_frame:
+                traffic = tj.get("bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "kernel": "render_kernel",

@@ -235,6 +235,41 @@ typedef struct svo_viewport {
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Batched mode (no counterpart in the reference, which renders one view    */
+/* per dispatch): many trees, viewports and ray arrays in one persistent    */
+/* launch. Waves pull 64-ray tiles from a device work counter until all     */
+/* tiles of all jobs are done. The launch sequence (job-table upload,       */
+/* counter reset, persistent kernel) is captured once in a hipGraph and     */
+/* replayed for every run.                                                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct svo_batch svo_batch;
+
+/* One job: kind 0 renders `viewport` at width x height (rays generated on
+ * the device, as svo_render); kind 1 casts `width` rays from `rays` (as
+ * svo_cast). Results go to `out` exactly as the single-job calls write them. */
+typedef struct svo_job {
+  const svo_tree* tree;
+  uint32_t kind;
+  uint32_t width;
+  uint32_t height;
+  svo_viewport viewport;
+  const float* rays;
+  svo_hits out;
+} svo_job;
+
+/* A reusable batch context for up to max_jobs jobs; its device buffers and
+ * graph live until svo_batch_destroy. Runs on one context are ordered by
+ * the caller; separate contexts may run concurrently on different streams. */
+int svo_batch_create(uint32_t max_jobs, svo_batch** out);
+void svo_batch_destroy(svo_batch* batch);
+/* Launches all jobs as one persistent kernel on `stream` (asynchronous). The
+ * caller's job array may be reused as soon as the call returns. */
+int svo_batch_run(svo_batch* batch, const svo_job* jobs, uint32_t n, void* stream);
+/* Total 64-ray tiles of the last run. */
+uint64_t svo_batch_tiles(const svo_batch* batch);
+
 /* Launch variants of the render kernel (no counterpart in the reference,
  * whose WGSL dispatch is fixed at 8x8 workgroups): occupancy hint, placement
  * of the occupancy LUT (global / LDS) and per-wave tile shape. The selection

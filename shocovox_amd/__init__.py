@@ -4,13 +4,13 @@ The traversal hot path (Octree::get_by_ray) runs as hand-written HIP
 kernels for gfx950 inside libsvo.so, behind the C ABI of include/svo.h.
 """
 from .octree import (  # noqa: F401
-    COMPLEX, EMPTY, EMPTY_MARKER, INFORMATIVE, VISUAL, DeviceTree, FlatTree, Octree,
+    COMPLEX, EMPTY, EMPTY_MARKER, INFORMATIVE, VISUAL, Batch, DeviceTree, FlatTree, Octree,
     OctreeEntry, OctreeError, Viewport, albedo, albedo_rgba, alloc_hits, collect_kernel_ms,
     current_render_variant, render_variants, select_render_variant, set_timing, voxel_data,
 )
 
 __all__ = [
-    "COMPLEX", "EMPTY", "EMPTY_MARKER", "INFORMATIVE", "VISUAL", "DeviceTree", "FlatTree",
+    "COMPLEX", "EMPTY", "EMPTY_MARKER", "INFORMATIVE", "VISUAL", "Batch", "DeviceTree", "FlatTree",
     "Octree", "OctreeEntry", "OctreeError", "Viewport", "albedo", "albedo_rgba", "alloc_hits",
     "collect_kernel_ms", "current_render_variant", "render_variants", "select_render_variant",
     "set_timing", "voxel_data",

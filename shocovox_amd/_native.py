@@ -62,6 +62,12 @@ class Viewport(C.Structure):
                 ("frustum", C.c_float * 3), ("fov", C.c_float)]
 
 
+class Job(C.Structure):
+    _fields_ = [("tree", C.c_void_p), ("kind", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("viewport", Viewport), ("rays", C.c_void_p),
+                ("out", Hits)]
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _SIGS = {
@@ -97,6 +103,10 @@ _SIGS = {
     "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),
     "svo_render": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.POINTER(Hits), _P]),
     "svo_ray_occupancy_lut": (C.c_int, [_P]),
+    "svo_batch_create": (C.c_int, [_U32, C.POINTER(_P)]),
+    "svo_batch_destroy": (None, [_P]),
+    "svo_batch_run": (C.c_int, [_P, _P, _U32, _P]),
+    "svo_batch_tiles": (C.c_uint64, [_P]),
     "svo_render_variant_count": (C.c_int, []),
     "svo_render_variant_name": (C.c_char_p, [C.c_int]),
     "svo_select_render_variant": (C.c_int, [C.c_int]),

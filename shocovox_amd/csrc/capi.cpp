@@ -2,8 +2,10 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/svo.h"
+#include "batch.hpp"
 #include "device.hpp"
 #include "flatten.hpp"
 #include "octree.hpp"
@@ -359,22 +361,59 @@ int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hit
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream) {
   if (!dtree || !vp || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
-  using svo::V3f;
-  const V3f origin{vp->origin[0], vp->origin[1], vp->origin[2]};
-  const V3f dir{vp->direction[0], vp->direction[1], vp->direction[2]};
-  const V3f up{0.0f, 1.0f, 0.0f};
-  svo::Camera cam;
-  cam.origin = origin;
-  cam.up = up;
-  cam.right = svo::normalized(svo::cross(up, dir));
-  cam.pixel_w = vp->frustum[0] / static_cast<float>(w);
-  cam.pixel_h = vp->frustum[1] / static_cast<float>(h);
-  cam.bottom_left = ((origin + dir * vp->fov) - up * (vp->frustum[1] / 2.0f)) -
-                    cam.right * (vp->frustum[0] / 2.0f);
+  const svo::Camera cam = svo::make_camera(
+      svo::V3f{vp->origin[0], vp->origin[1], vp->origin[2]},
+      svo::V3f{vp->direction[0], vp->direction[1], vp->direction[2]}, vp->frustum[0],
+      vp->frustum[1], vp->fov, w, h);
   std::string err;
   const int st = svo::device_render(*dtree->dev, cam, w, h, *out, stream, &err);
   if (st != SVO_OK) return fail(st, err);
   return SVO_OK;
+}
+
+int svo_batch_create(uint32_t max_jobs, svo_batch** out) {
+  if (!out) return fail(SVO_INVALID_ARGUMENT, "out is NULL");
+  svo::BatchContext* b = nullptr;
+  std::string err;
+  const int st = svo::batch_create(max_jobs, &b, &err);
+  if (st != SVO_OK) return fail(st, err);
+  *out = reinterpret_cast<svo_batch*>(b);
+  return SVO_OK;
+}
+
+void svo_batch_destroy(svo_batch* batch) {
+  svo::batch_destroy(reinterpret_cast<svo::BatchContext*>(batch));
+}
+
+int svo_batch_run(svo_batch* batch, const svo_job* jobs, uint32_t n, void* stream) {
+  if (!batch || (n && !jobs)) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::vector<svo::BatchJobIn> in(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const svo_job& j = jobs[i];
+    if (!j.tree || !j.tree->dev) return fail(SVO_INVALID_ARGUMENT, "job without a tree");
+    if (j.kind > 1) return fail(SVO_INVALID_ARGUMENT, "unknown job kind");
+    svo::BatchJobIn& b = in[i];
+    b.tree = j.tree->dev;
+    b.kind = j.kind;
+    b.width = j.width;
+    b.height = j.height;
+    b.rays = j.rays;
+    b.out = j.out;
+    if (j.kind == 0)
+      b.cam = svo::make_camera(
+          svo::V3f{j.viewport.origin[0], j.viewport.origin[1], j.viewport.origin[2]},
+          svo::V3f{j.viewport.direction[0], j.viewport.direction[1], j.viewport.direction[2]},
+          j.viewport.frustum[0], j.viewport.frustum[1], j.viewport.fov, j.width, j.height);
+  }
+  std::string err;
+  const int st = svo::batch_run(reinterpret_cast<svo::BatchContext*>(batch), in.data(), n, stream,
+                                &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
+uint64_t svo_batch_tiles(const svo_batch* batch) {
+  return svo::batch_tiles(reinterpret_cast<const svo::BatchContext*>(batch));
 }
 
 int svo_ray_occupancy_lut(uint64_t* out512) {

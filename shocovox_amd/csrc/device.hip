@@ -11,59 +11,12 @@
 #include <vector>
 
 #include "device.hpp"
+#include "device_common.hpp"
 #include "flatten.hpp"
 
 namespace svo {
 
 namespace {
-
-struct HitPtrs {
-  uint32_t* hit;
-  uint32_t* pix;
-  uint32_t* albedo;
-  uint32_t* voxel;
-  float* distance;
-  float* impact;
-  float* normal;
-  uint32_t* flags;
-};
-
-HitPtrs to_ptrs(const svo_hits& h) {
-  return HitPtrs{h.hit, h.pix, h.albedo, h.voxel, h.distance, h.impact, h.normal, h.flags};
-}
-
-__device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint32_t i,
-                                    const CastResult& r, V3f org) {
-  if (o.hit) o.hit[i] = r.hit;
-  if (o.pix) o.pix[i] = r.hit ? r.pix : EMPTY_MARKER;
-  if (o.albedo) {
-    const uint32_t c = pix_color_index(r.pix);
-    o.albedo[i] = (r.hit && c < NONE16 && c < t.color_count) ? t.colors[c] : 0u;
-  }
-  if (o.voxel) {
-    if (r.hit) {
-      o.voxel[3 * i + 0] = hit_voxel_axis(r.impact.x, r.cell.min.x, r.cell.size);
-      o.voxel[3 * i + 1] = hit_voxel_axis(r.impact.y, r.cell.min.y, r.cell.size);
-      o.voxel[3 * i + 2] = hit_voxel_axis(r.impact.z, r.cell.min.z, r.cell.size);
-    } else {
-      o.voxel[3 * i + 0] = EMPTY_MARKER;
-      o.voxel[3 * i + 1] = EMPTY_MARKER;
-      o.voxel[3 * i + 2] = EMPTY_MARKER;
-    }
-  }
-  if (o.distance) o.distance[i] = r.hit ? length(r.impact - org) : __builtin_inff();
-  if (o.impact) {
-    o.impact[3 * i + 0] = r.impact.x;
-    o.impact[3 * i + 1] = r.impact.y;
-    o.impact[3 * i + 2] = r.impact.z;
-  }
-  if (o.normal) {
-    o.normal[3 * i + 0] = r.normal.x;
-    o.normal[3 * i + 1] = r.normal.y;
-    o.normal[3 * i + 2] = r.normal.z;
-  }
-  if (o.flags) o.flags[i] = r.flags;
-}
 
 __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __restrict__ rays,
                                                    uint32_t n, HitPtrs out) {

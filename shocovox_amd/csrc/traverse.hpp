@@ -284,6 +284,22 @@ struct Camera {
   float pixel_h;
 };
 
+// Camera of a Viewport{origin, direction, frustum, fov} (bevy/types.rs:55-71)
+// as the reference's CPU harness derives it (examples/cpu_render.rs:78-94):
+// up = +y, right = normalize(up x direction), glass `fov` in front of the eye.
+SVO_HD inline Camera make_camera(V3f origin, V3f dir, float glass_w, float glass_h, float fov,
+                                 uint32_t w, uint32_t h) {
+  const V3f up{0.0f, 1.0f, 0.0f};
+  Camera cam;
+  cam.origin = origin;
+  cam.up = up;
+  cam.right = normalized(cross(up, dir));
+  cam.pixel_w = glass_w / static_cast<float>(w);
+  cam.pixel_h = glass_h / static_cast<float>(h);
+  cam.bottom_left = ((origin + dir * fov) - up * (glass_h / 2.0f)) - cam.right * (glass_w / 2.0f);
+  return cam;
+}
+
 SVO_HD inline V3f camera_ray_dir(const Camera& c, uint32_t x, uint32_t y) {
   const float fx = static_cast<float>(x) + 0.5f;
   const float fy = static_cast<float>(y) + 0.5f;

@@ -368,6 +368,41 @@ class DeviceTree:
         return bufs
 
 
+class Batch:
+    """Batched mode: many (tree, viewport | rays) jobs in one persistent launch
+    captured in a hipGraph (svo_batch_*)."""
+
+    def __init__(self, max_jobs: int):
+        h = C.c_void_p()
+        _check(N.lib().svo_batch_create(max_jobs, C.byref(h)))
+        self._h = h.value
+        self.max_jobs = max_jobs
+        self._keep = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib().svo_batch_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def render_job(tree: "DeviceTree", vp: "Viewport", w: int, h: int, bufs: dict) -> N.Job:
+        return N.Job(tree._h, 0, w, h, vp.to_c(), None, _hits_struct(bufs))
+
+    @staticmethod
+    def cast_job(tree: "DeviceTree", rays, bufs: dict) -> N.Job:
+        return N.Job(tree._h, 1, rays.shape[0], 1, N.Viewport(), rays.data_ptr(),
+                     _hits_struct(bufs))
+
+    def run(self, jobs: list, stream=None) -> None:
+        arr = (N.Job * max(1, len(jobs)))(*jobs)
+        _check(N.lib().svo_batch_run(self._h, arr, len(jobs), _stream_ptr(stream)))
+
+    @property
+    def tiles(self) -> int:
+        return int(N.lib().svo_batch_tiles(self._h))
+
+
 def render_variants() -> list:
     """Names of the render kernel's launch variants, in index order."""
     L = N.lib()

@@ -61,14 +61,9 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out) {
 
 void hs_viewport_rays(const float* vp, uint32_t w, uint32_t h, float* rays) {
   using svo::V3f;
-  const V3f origin{vp[0], vp[1], vp[2]}, dir{vp[3], vp[4], vp[5]}, up{0.f, 1.f, 0.f};
-  svo::Camera cam;
-  cam.origin = origin;
-  cam.up = up;
-  cam.right = svo::normalized(svo::cross(up, dir));
-  cam.pixel_w = vp[6] / static_cast<float>(w);
-  cam.pixel_h = vp[7] / static_cast<float>(h);
-  cam.bottom_left = ((origin + dir * vp[9]) - up * (vp[7] / 2.0f)) - cam.right * (vp[6] / 2.0f);
+  const V3f origin{vp[0], vp[1], vp[2]};
+  const svo::Camera cam =
+      svo::make_camera(origin, V3f{vp[3], vp[4], vp[5]}, vp[6], vp[7], vp[9], w, h);
   for (uint32_t y = 0; y < h; ++y)
     for (uint32_t x = 0; x < w; ++x) {
       const V3f d = svo::camera_ray_dir(cam, x, y);

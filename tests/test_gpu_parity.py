@@ -223,3 +223,55 @@ def test_in_place_edits_on_device(gpu_device):
         torch.cuda.synchronize()
         exp = _oracle(tree).cast(rays)
         assert_parity(exp, gpu_to_numpy(bufs), f"edit round {rnd}")
+
+
+def test_batched_mixed_jobs_and_replays(gpu_device):
+    """Many trees, uneven viewports and ray arrays in one persistent launch;
+    the captured graph is replayed with different job sets (with eager work in
+    between) and every job of every replay equals the oracle."""
+    import torch
+    from shocovox_amd import Batch, Octree, alloc_hits
+    rng = np.random.default_rng(23)
+    trees = [W.build_scene(W.PLAIN, seed=s) for s in (1, 2)]
+    small = Octree.new(64, 4)
+    xyz = rng.integers(0, 64, size=(5000, 3)).astype(np.uint32)
+    ents = np.zeros((5000, 3), np.uint32)
+    ents[:, 0] = 1
+    ents[:, 1] = 0x3355AAFF
+    small.insert_many(xyz, ents)
+    trees.append(small)
+    dtrees = [DeviceTree(t) for t in trees]
+    oracles = [_oracle(t) for t in trees]
+    batch = Batch(16)
+    for replay in range(3):
+        jobs, checks = [], []
+        for k in range(int(rng.integers(3, 9))):
+            ti = int(rng.integers(0, len(trees)))
+            size = trees[ti].get_size()
+            if rng.random() < 0.6:
+                w, h = int(rng.integers(1, 200)), int(rng.integers(1, 150))
+                c = size / 2
+                vp = W.look_at_viewport((c + size * 1.3 * np.cos(k), size * 0.9, c + size * np.sin(k)),
+                                        (c, c / 2, c), w, h)
+                bufs = alloc_hits(w * h, gpu_device)
+                jobs.append(Batch.render_job(dtrees[ti], vp, w, h, bufs))
+                rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, w, h)
+            else:
+                n = int(rng.integers(1, 3000))
+                rays = W.incoherent_rays(size, n, seed=int(rng.integers(0, 1 << 30)))
+                t = torch.from_numpy(rays).to(gpu_device)
+                bufs = alloc_hits(n, gpu_device)
+                jobs.append(Batch.cast_job(dtrees[ti], t, bufs))
+                bufs["_rays"] = t
+            checks.append((ti, rays, bufs))
+        _say(f"batched replay {replay}: {len(jobs)} jobs")
+        batch.run(jobs)
+        torch.cuda.synchronize()
+        # unrelated eager work between replays
+        torch.ones(1 << 20, device=gpu_device).sum().item()
+        for ti, rays, bufs in checks:
+            got = gpu_to_numpy({k: v for k, v in bufs.items() if k != "_rays"})
+            assert_parity(oracles[ti].cast(rays), got, f"replay {replay} tree {ti}")
+    # an empty batch is valid
+    batch.run([])
+    torch.cuda.synchronize()
