@@ -108,6 +108,173 @@ def copy_calibration(args) -> int:
     return 0
 
 
+BATCH_RESOLUTIONS = ((128, 128), (512, 512), (1280, 720), (1920, 1080))
+
+
+def batched_bench(args, render_variant: str) -> dict:
+    """Batched mode: trees of uneven size (log-uniform 64..max), each with 1-4
+    viewports of resolution drawn from {128^2, 512^2, 720p, 1080p}; all rays in
+    one persistent launch per batch kernel variant (a replayed hipGraph),
+    against one plain render launch per viewport. Outputs of the two forms are
+    compared for equality."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from shocovox_amd import (Batch, DeviceTree, Octree, alloc_hits, batch_variants,
+                              select_batch_variant, workloads as W)
+
+    rng = np.random.default_rng(args.seed + 700)
+    max_exp = int(np.log2(args.batched_max_size))
+    t0 = time.time()
+    trees, dtrees, views = [], [], []
+    for i in range(args.batched_trees):
+        size = int(2 ** rng.integers(6, max_exp + 1))
+        dim = int(rng.choice([4, 8, 16]))
+        kind = int(rng.integers(1, 3)) if size <= 128 else 2
+        t = Octree.scene(kind, size, dim, args.seed * 1000 + 500 + i)
+        trees.append(t)
+        dtrees.append(DeviceTree(t))
+        c = size / 2.0
+        for _ in range(int(rng.integers(1, 5))):
+            w, h = BATCH_RESOLUTIONS[int(rng.integers(0, len(BATCH_RESOLUTIONS)))]
+            az = float(rng.uniform(0, 2 * math.pi))
+            eye = (c + 1.3 * size * math.cos(az), 0.9 * size, c + 1.3 * size * math.sin(az))
+            views.append((i, W.look_at_viewport(eye, (c, c / 2, c), w, h), w, h))
+    rays = sum(w * h for _, _, w, h in views)
+    log(f"batched: {len(trees)} trees, {len(views)} viewports, {rays} rays, built in "
+        f"{time.time() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    fields = ("hit", "albedo", "voxel", "distance")
+    plain_bufs = [alloc_hits(w * h, dev, fields) for _, _, w, h in views]
+    batch_bufs = [alloc_hits(w * h, dev, fields) for _, _, w, h in views]
+    stream = torch.cuda.current_stream()
+    reps = max(1, args.batched_reps)
+
+    def plain_pass():
+        for (ti, vp, w, h), b in zip(views, plain_bufs):
+            dtrees[ti].render_into(vp, w, h, b, stream)
+
+    plain_pass()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    torch.cuda.synchronize()
+    plain_s = (time.perf_counter() - t1) / reps
+    out = {"workload": (f"{len(trees)} trees (sizes 64..{args.batched_max_size}, brick dims "
+                        f"4/8/16), {len(views)} viewports of 128^2/512^2/720p/1080p"),
+           "trees": len(trees), "viewports": len(views), "rays": rays,
+           "plain": {"ms": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 1),
+                     "launches": len(views), "render_variant": render_variant},
+           "batched": {}}
+    names = batch_variants()
+    best = None
+    for vi, name in enumerate(names):
+        select_batch_variant(vi)
+        batch = Batch(len(views))
+        jobs = [Batch.render_job(dtrees[ti], vp, w, h, b)
+                for (ti, vp, w, h), b in zip(views, batch_bufs)]
+        batch.run(jobs, stream)
+        torch.cuda.synchronize()
+        same = all(torch.equal(a[k], b[k]) for a, b in zip(plain_bufs, batch_bufs) for k in fields)
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            batch.run(jobs, stream)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t1) / reps
+        out["batched"][name] = {"ms": round(el * 1e3, 3), "mrays_per_s": round(rays / el / 1e6, 1),
+                                "equal_to_plain": bool(same)}
+        log(f"batched {name}: {el * 1e3:.3f} ms, {rays / el / 1e6:.1f} Mrays/s, equal {same}")
+        for b in batch_bufs:
+            for k in fields:
+                b[k].zero_()
+        if same and (best is None or el < out["batched"][best]["ms"] / 1e3):
+            best = name
+        del batch
+    select_batch_variant(names.index(best) if best else 0)
+    out["best_variant"] = best
+    if best:
+        out["speedup_vs_plain"] = round(plain_s / (out["batched"][best]["ms"] / 1e3), 3)
+    return out
+
+
+def served_bench(args, capacity_mrays: float) -> dict:
+    """Served mode: a pool of resident trees admitted by free HBM, then the
+    same seeded Poisson plan (Zipf(1.1) over trees and resolutions, 5% sparse
+    edits of 1-1000 voxels) served by continuous batching and by per-request
+    plain launches, at fixed fractions of the measured render capacity."""
+    import numpy as np
+
+    from shocovox_amd import Octree, batch_variants, current_batch_variant, serve
+
+    rng = np.random.default_rng(args.seed + 500)
+    max_exp = int(np.log2(args.served_max_size))
+    specs = []
+    for i in range(args.served_trees):
+        size = int(2 ** rng.integers(6, max_exp + 1))  # log-uniform over 64..max
+        dim = int(rng.choice([4, 8, 16]))
+        kind = int(rng.integers(1, 3)) if size <= 128 else 2
+        specs.append((kind, size, dim, int(args.seed * 1000 + i)))
+    cfg0 = serve.ServeConfig(qps=1.0, duration_s=1.0, seed=args.seed,
+                             max_inflight=args.served_inflight,
+                             max_batch_rays=args.served_batch_rays)
+    # device memory kept free for the loop's output pools (either mode) and a margin
+    reserve = (cfg0.max_inflight * max(cfg0.max_batch_rays, 3840 * 2160) * 24) + (1 << 30)
+    t0 = time.time()
+    trees, dtrees, refused, pool_bytes = [], [], 0, 0
+    for kind, size, dim, seed in specs:
+        t = Octree.scene(kind, size, dim, seed)
+        try:
+            d = serve.admit(t, reserve_bytes=reserve)
+        except Exception as e:  # admission refused: the tree stays on the host
+            log(f"served: tree {size}^3 refused: {e}")
+            refused += 1
+            continue
+        trees.append(t)
+        dtrees.append(d)
+        pool_bytes += d.device_bytes
+    log(f"served: {len(trees)} trees resident ({pool_bytes / 2**30:.2f} GiB) in "
+        f"{time.time() - t0:.1f}s, {refused} refused")
+    sizes = [t.get_size() for t in trees]
+    probe = serve.plan(serve.ServeConfig(qps=1000.0, duration_s=20.0, seed=args.seed), sizes)
+    mean_rays = sum(r.width * r.height for r in probe if r.kind == 0) / len(probe)
+    qps_cap = capacity_mrays * 1e6 / mean_rays
+    levels = []
+    for frac in args.served_loads:
+        qps = round(frac * qps_cap, 1)
+        row = {"load": frac, "qps": qps}
+        for mode in ("batched", "plain"):
+            cfg = serve.ServeConfig(qps=qps, duration_s=args.served_seconds, seed=args.seed,
+                                    mode=mode, max_inflight=args.served_inflight,
+                                    max_batch_rays=args.served_batch_rays)
+            # sessions run back to back on the same pool; their sparse edits accumulate
+            r = serve.run(trees, dtrees, cfg)
+            row[mode] = {"mrays_per_s": round(r["mrays_per_s"], 1),
+                         "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
+                         "render_p99_ms": round(r["render_p99_ms"], 3),
+                         "edit_p99_ms": round(r["edit_p99_ms"], 3),
+                         "requests": int(r["requests"]), "edits": int(r["edits"]),
+                         "launches": int(r["launches"]),
+                         "jobs_per_launch": round(r["mean_jobs_per_launch"], 2),
+                         "elapsed_s": round(r["elapsed_s"], 3)}
+            log(f"served load {frac} ({qps} req/s) {mode}: {row[mode]}")
+        b, p = row["batched"], row["plain"]
+        row["throughput_ratio"] = round(b["mrays_per_s"] / max(p["mrays_per_s"], 1e-9), 3)
+        levels.append(row)
+    return {
+        "workload": (f"{len(trees)} resident trees (sizes {min(sizes)}..{max(sizes)}, brick dims "
+                     "4/8/16), Poisson arrivals, Zipf(1.1) over tree and resolution "
+                     "{256^2, 720p, 1080p, 4K}, 5% edits of 1..1000 voxels"),
+        "trees": len(trees), "refused": refused, "pool_bytes": pool_bytes,
+        "mean_rays_per_request": round(mean_rays), "capacity_qps": round(qps_cap, 1),
+        "max_inflight": args.served_inflight, "max_batch_rays": args.served_batch_rays,
+        "batch_variant": batch_variants()[current_batch_variant()],
+        "arrival_seconds": args.served_seconds, "levels": levels,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +287,18 @@ def main() -> int:
     ap.add_argument("--variant", default="auto",
                     help="render launch variant name or index; 'auto' times each and keeps "
                          "the fastest before the timed region")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-mode comparison")
+    ap.add_argument("--batched-trees", type=int, default=64)
+    ap.add_argument("--batched-max-size", type=int, default=512)
+    ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--no-served", action="store_true", help="skip the served-mode sessions")
+    ap.add_argument("--served-trees", type=int, default=32)
+    ap.add_argument("--served-max-size", type=int, default=512)
+    ap.add_argument("--served-seconds", type=float, default=2.0)
+    ap.add_argument("--served-inflight", type=int, default=2)
+    ap.add_argument("--served-batch-rays", type=int, default=1 << 21)
+    ap.add_argument("--served-loads", type=float, nargs="+", default=[0.3, 0.6, 0.9, 1.2],
+                    help="offered load as fractions of the measured render capacity")
     ap.add_argument("--mode", default="render", choices=["render", "copy"],
                     help="'copy' runs a plain device-to-device copy of known size (for "
                          "calibrating rocprofv3 byte counters)")
@@ -234,6 +413,10 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.no_batched:
+        line["batched"] = batched_bench(args, best)
+    if not args.no_served:
+        line["served"] = served_bench(args, value)
     print(json.dumps(line), flush=True)
     return 0
 

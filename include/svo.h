@@ -235,6 +235,22 @@ typedef struct svo_viewport {
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream);
 
+/* Shaded render to RGBA8, the drop-in for the image the reference draws:
+ * out holds w*h u32 pixels (r in the low byte, alpha 255), row 0 at the top.
+ * convention 0: examples/cpu_render.rs:78-137 - rays from the eye through
+ *   pixel corners of the glass (bottom_left + right*x*pixel_w +
+ *   up*y*pixel_h, row h-1-y), hit colour albedo * (1 - (n.L/2 + 0.5)) with
+ *   L = normalize(0,-1,1) cast per channel like `as u8`, miss (128,128,128);
+ * convention 1: the output texture of OctreeGPUHost::create_new_view, as
+ *   viewport_render.wgsl:1028-1089 fills it - rays from the glass point
+ *   (x/W8 and 1 - row/H8 across the glass, W8/H8 rounded up to 8) away from
+ *   the eye, hit colour albedo * (n.(-0.5,0.5,-0.5)/2 + 0.5), miss 0.25 grey,
+ *   unorm8 = floor(clamp(v,0,1)*255 + 0.5).
+ * Traversal is get_by_ray as everywhere else; the viewport is read as in
+ * svo_render. */
+int svo_render_rgba8(const svo_tree* tree, const svo_viewport* viewport, uint32_t width,
+                     uint32_t height, uint32_t convention, uint32_t* out, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Batched mode (no counterpart in the reference, which renders one view    */
 /* per dispatch): many trees, viewports and ray arrays in one persistent    */
@@ -269,6 +285,107 @@ void svo_batch_destroy(svo_batch* batch);
 int svo_batch_run(svo_batch* batch, const svo_job* jobs, uint32_t n, void* stream);
 /* Total 64-ray tiles of the last run. */
 uint64_t svo_batch_tiles(const svo_batch* batch);
+/* Kernel variants of batched launches (no counterpart in the reference):
+ * "classic" (one 64-ray tile per work grab, one ray per lane) and lane-refill
+ * forms (several tiles per grab; finished lanes start the next rays of the
+ * grab). A context keeps the variant selected when it was created; results
+ * are identical across variants. */
+int svo_batch_variant_count(void);
+const char* svo_batch_variant_name(int variant);
+int svo_select_batch_variant(int variant);
+int svo_current_batch_variant(void);
+
+/* ---------------------------------------------------------------------- */
+/* Serving loop (no counterpart in the reference): ray-cast requests arrive */
+/* as a seeded Poisson process; each picks a tree and a resolution by Zipf; */
+/* a share of requests are sparse voxel edits applied in place.             */
+/*  mode 0, continuous batching: at every launch boundary the arrived       */
+/*    requests are packed, in arrival order, into one batched launch of at  */
+/*    most max_batch_rays rays; a large request is cut into 8x8-tile chunks */
+/*    that run in consecutive launches, so requests that arrive meanwhile   */
+/*    join the next launch; up to max_inflight launches are queued on their */
+/*    own streams.                                                          */
+/*  mode 1, per-request plain launches: each render request is one          */
+/*    svo_render launch, in arrival order, on up to max_inflight streams.  */
+/* Both modes keep per-tree order: a render sees exactly the edits that     */
+/* arrived before it. Edits run on a host worker thread (host octree edit,  */
+/* then svo_tree_update on its own stream) once no earlier render of the    */
+/* tree is queued or running. Outputs of a render (hit, albedo, voxel,      */
+/* distance) stay in device memory owned by the loop until its slot is      */
+/* reused.                                                                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct svo_serve_config {
+  double qps;               /* mean arrival rate, requests per second */
+  double duration_s;        /* arrivals are generated over [0, duration_s) */
+  uint64_t seed;
+  uint32_t mode;            /* 0 continuous batching, 1 per-request launches */
+  uint32_t max_inflight;    /* launches queued at once (1..16) */
+  uint64_t max_batch_rays;  /* ray budget of one batched launch (mode 0) */
+  uint32_t max_batch_jobs;  /* chunk budget of one batched launch (mode 0) */
+  uint32_t max_queue;       /* arrivals finding this many queued are rejected; 0 = no limit */
+  double edit_fraction;     /* share of requests that are edits */
+  uint32_t max_edit_voxels; /* an edit inserts or clears 1..max_edit_voxels voxels */
+  double zipf_s;            /* Zipf exponent of the tree and resolution choice */
+  uint32_t n_resolutions;
+  const uint32_t* resolutions; /* n_resolutions (width, height) pairs, most popular first */
+} svo_serve_config;
+
+/* One request of a serving plan. Render viewports look at the tree's centre
+ * from outside it (60 degree vertical fov). An edit inserts (random palette
+ * colour) or clears edit_voxels uniformly drawn voxels from edit_seed. */
+typedef struct svo_serve_request {
+  double arrival_s;
+  uint32_t kind; /* 0 render, 1 edit */
+  uint32_t tree;
+  uint32_t width;
+  uint32_t height;
+  svo_viewport viewport;
+  uint32_t edit_voxels;
+  uint32_t edit_clear; /* 1 clear, 0 insert */
+  uint64_t edit_seed;
+} svo_serve_request;
+
+typedef struct svo_serve_stats {
+  uint64_t requests, renders, edits, rejected, rays, launches;
+  double elapsed_s;         /* first arrival to last completion */
+  double mrays_per_s;       /* rays of completed renders / elapsed_s */
+  double p50_ms, p99_ms, mean_ms, max_ms; /* latency of completed requests */
+  double render_p50_ms, render_p99_ms;    /* render requests only */
+  double edit_p50_ms, edit_p99_ms;        /* edit requests only */
+  double mean_jobs_per_launch;
+  uint64_t edit_records, edit_bricks;     /* node records / brick slots uploaded by edits */
+} svo_serve_stats;
+
+/* The request sequence of a serving session (deterministic in cfg and the
+ * tree sizes). Call with out == NULL to learn the count. */
+int svo_serve_plan(const svo_serve_config* cfg, const uint32_t* tree_sizes, uint32_t n_trees,
+                   svo_serve_request* out, uint64_t cap, uint64_t* count);
+
+/* Applies an edit request of a plan to a host octree (what the loop does). */
+int svo_serve_apply_edit(svo_octree* tree, const svo_serve_request* req);
+
+/* Runs one serving session in real time over n_trees resident trees (each a
+ * host octree and its device tree; edits change both). latencies_ms and
+ * digests, when not NULL, receive one value per request of the plan
+ * (capacity cap): latency (negative when rejected) and, for renders, a
+ * 64-bit digest of the result: the wrapping sum over pixels i = y*w+x of
+ * m(...m(m(m(i + 0x9E3779B97F4A7C15) ^ hit) ^ albedo) ... ^ distance_bits)
+ * over hit, albedo, voxel x/y/z and the f32 bits of distance, where m is
+ * the splitmix64 finaliser. Digests add a small kernel per launch. */
+int svo_serve_run(svo_octree* const* trees, svo_tree* const* dtrees, uint32_t n_trees,
+                  const svo_serve_config* cfg, svo_serve_stats* stats, float* latencies_ms,
+                  uint64_t* digests, uint64_t cap);
+
+/* Free and total bytes of device memory on the current device. */
+int svo_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* Admission by free device memory: flattens the octree and uploads it only
+ * if its device bytes plus reserve_bytes fit in the free memory (queried
+ * when budget_bytes is 0, else budget_bytes stands for it); otherwise
+ * returns SVO_OUT_OF_DEVICE_MEMORY without touching the device. */
+int svo_tree_admit(svo_octree* tree, uint64_t budget_bytes, uint64_t reserve_bytes,
+                   svo_tree** out);
 
 /* Launch variants of the render kernel (no counterpart in the reference,
  * whose WGSL dispatch is fixed at 8x8 workgroups): occupancy hint, placement

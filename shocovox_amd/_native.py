@@ -68,8 +68,35 @@ class Job(C.Structure):
                 ("out", Hits)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("qps", C.c_double), ("duration_s", C.c_double), ("seed", C.c_uint64),
+                ("mode", C.c_uint32), ("max_inflight", C.c_uint32),
+                ("max_batch_rays", C.c_uint64), ("max_batch_jobs", C.c_uint32),
+                ("max_queue", C.c_uint32), ("edit_fraction", C.c_double),
+                ("max_edit_voxels", C.c_uint32), ("zipf_s", C.c_double),
+                ("n_resolutions", C.c_uint32), ("resolutions", C.POINTER(C.c_uint32))]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("kind", C.c_uint32), ("tree", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("viewport", Viewport),
+                ("edit_voxels", C.c_uint32), ("edit_clear", C.c_uint32),
+                ("edit_seed", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = ([(k, C.c_uint64) for k in ("requests", "renders", "edits", "rejected", "rays",
+                                            "launches")]
+                + [(k, C.c_double) for k in ("elapsed_s", "mrays_per_s", "p50_ms", "p99_ms",
+                                             "mean_ms", "max_ms", "render_p50_ms",
+                                             "render_p99_ms", "edit_p50_ms", "edit_p99_ms",
+                                             "mean_jobs_per_launch")]
+                + [("edit_records", C.c_uint64), ("edit_bricks", C.c_uint64)])
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
+_U64 = C.c_uint64
 _SIGS = {
     "svo_last_error": (C.c_char_p, []),
     "svo_version": (C.c_char_p, []),
@@ -102,11 +129,23 @@ _SIGS = {
     "svo_tree_device_bytes": (C.c_uint64, [_P]),
     "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),
     "svo_render": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.POINTER(Hits), _P]),
+    "svo_render_rgba8": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, _U32, _P, _P]),
     "svo_ray_occupancy_lut": (C.c_int, [_P]),
     "svo_batch_create": (C.c_int, [_U32, C.POINTER(_P)]),
     "svo_batch_destroy": (None, [_P]),
     "svo_batch_run": (C.c_int, [_P, _P, _U32, _P]),
     "svo_batch_tiles": (C.c_uint64, [_P]),
+    "svo_batch_variant_count": (C.c_int, []),
+    "svo_batch_variant_name": (C.c_char_p, [C.c_int]),
+    "svo_select_batch_variant": (C.c_int, [C.c_int]),
+    "svo_current_batch_variant": (C.c_int, []),
+    "svo_serve_plan": (C.c_int, [C.POINTER(ServeConfig), _P, _U32, _P, _U64,
+                                 C.POINTER(_U64)]),
+    "svo_serve_apply_edit": (C.c_int, [_P, C.POINTER(ServeRequest)]),
+    "svo_serve_run": (C.c_int, [_P, _P, _U32, C.POINTER(ServeConfig), C.POINTER(ServeStats), _P,
+                                _P, _U64]),
+    "svo_device_memory": (C.c_int, [C.POINTER(_U64), C.POINTER(_U64)]),
+    "svo_tree_admit": (C.c_int, [_P, _U64, _U64, C.POINTER(_P)]),
     "svo_render_variant_count": (C.c_int, []),
     "svo_render_variant_name": (C.c_char_p, [C.c_int]),
     "svo_select_render_variant": (C.c_int, [C.c_int]),

@@ -5,13 +5,19 @@
 // contiguously across jobs. The tile space is split into 8 contiguous
 // chunks, one per XCD (workgroups are dispatched to the 8 XCDs round-robin),
 // so neighbouring tiles - which touch the same nodes and bricks - are served
-// by one XCD's L2. Each wave takes the next tile of its XCD's chunk from a
+// by one XCD's L2. Each wave takes the next tiles of its XCD's chunk from a
 // device counter and, once that chunk is exhausted, steals from the other
 // chunks; every wave leaves when all chunks are empty.
 //
-// The per-run sequence - job table upload from pinned memory, counter reset
-// (a fill kernel, not a memset), persistent kernel - is captured once in a
-// hipGraph and replayed for each run.
+// Kernel variants: "classic" takes one tile per grab and runs it as one ray
+// per lane (with an occupancy hint of 1 or 5 waves per SIMD); the refill
+// variants take TPW tiles per grab and run them with lane refill
+// (refill.hpp), one job segment at a time.
+//
+// The per-run sequence - job table upload from pinned memory (only the bytes
+// the run uses), counter reset (a fill kernel, not a memset), persistent
+// kernel - is a hipGraph built once per context; each run sets the copy size
+// and replays it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +28,7 @@
 
 #include "batch.hpp"
 #include "device_common.hpp"
+#include "refill.hpp"
 
 namespace svo {
 
@@ -30,25 +37,29 @@ namespace {
 constexpr uint32_t kChunks = 8;  // one per XCD of an MI355X
 
 struct alignas(16) BatchJob {
-  uint32_t tree;        // index into the batch's tree table
+  uint32_t tree;        // index into the run's tree table
   uint32_t kind;        // 0 render, 1 cast
   uint32_t width;       // render width or ray count
   uint32_t height;      // render height (1 for casts)
-  uint32_t tiles_x;     // render: tiles per row
+  uint32_t tiles_x;     // render: tiles per image row
   uint32_t tile_begin;  // first global tile of the job
   uint32_t tile_count;
-  uint32_t pad;
+  uint32_t first_tile;  // render: first image tile of the job
+  uint32_t tile_major;  // 1: outputs stored tile after tile from out[0]
+  uint32_t pad[3];
   Camera cam;
   const float* rays;
   HitPtrs out;
 };
 
+// Table of one run: [header][jobs x njobs][trees x ntrees]
 struct alignas(16) BatchHeader {
   uint32_t njobs;
   uint32_t total_tiles;
-  uint32_t chunk_tiles;  // tiles per XCD chunk (the last chunk may be shorter)
-  uint32_t pad;
+  uint32_t chunk_tiles;   // tiles per XCD chunk (the last chunk may be shorter)
+  uint32_t trees_offset;  // byte offset of the tree table
 };
+constexpr size_t kJobsOffset = 256;
 
 struct alignas(128) Counter {
   uint32_t next;
@@ -72,10 +83,22 @@ __global__ __launch_bounds__(64) void reset_counters_kernel(Counter* counters) {
   if (threadIdx.x < kChunks) counters[threadIdx.x].next = 0u;
 }
 
-__global__ __launch_bounds__(256) void batch_kernel(const BatchHeader* __restrict__ hdr,
-                                                    const TreeView* __restrict__ trees,
-                                                    const BatchJob* __restrict__ jobs,
-                                                    Counter* __restrict__ counters) {
+__device__ inline const BatchHeader* header_of(const char* table) {
+  return reinterpret_cast<const BatchHeader*>(table);
+}
+__device__ inline const BatchJob* jobs_of(const char* table) {
+  return reinterpret_cast<const BatchJob*>(table + kJobsOffset);
+}
+__device__ inline const TreeView* trees_of(const char* table) {
+  return reinterpret_cast<const TreeView*>(table + header_of(table)->trees_offset);
+}
+
+template <int MIN_WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
+void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters) {
+  const BatchHeader* hdr = header_of(table);
+  const BatchJob* jobs = jobs_of(table);
+  const TreeView* trees = trees_of(table);
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t njobs = hdr->njobs;
   const uint32_t total = hdr->total_tiles;
@@ -89,84 +112,180 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchHeader* __restric
     for (;;) {
       uint32_t t = 0;
       if (lane == 0) t = atomicAdd(&counters[c].next, 1u);
-      t = __shfl(t, 0, 64);
+      // wave-uniform from here: the job and its tree view load into SGPRs
+      t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+      if (t >= end - begin) break;
       const uint32_t tile = begin + t;
-      if (tile >= end) break;
-      const uint32_t j = find_job(jobs, njobs, tile);
+      const uint32_t j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, tile));
       const BatchJob& job = jobs[j];
       const TreeView& tv = trees[job.tree];
       const uint32_t local = tile - job.tile_begin;
+      // the lane's ray; one inlined traversal serves both job kinds
+      V3f o, d;
+      uint32_t idx;
+      bool valid;
       if (job.kind == 0) {
-        const uint32_t x = (local % job.tiles_x) * 8u + (lane & 7u);
-        const uint32_t y = (local / job.tiles_x) * 8u + (lane >> 3);
-        if (x < job.width && y < job.height) {
-          const V3f d = camera_ray_dir(job.cam, x, y);
-          const CastResult r = cast_ray(tv, job.cam.origin, d);
-          store_result(tv, job.out, y * job.width + x, r, job.cam.origin);
-        }
+        const uint32_t image_tile = local + job.first_tile;
+        const uint32_t x = (image_tile % job.tiles_x) * 8u + (lane & 7u);
+        const uint32_t y = (image_tile / job.tiles_x) * 8u + (lane >> 3);
+        valid = x < job.width && y < job.height;
+        o = job.cam.origin;
+        d = valid ? camera_ray_dir(job.cam, x, y) : V3f{0.0f, 0.0f, 0.0f};
+        idx = job.tile_major ? local * 64u + lane : y * job.width + x;
       } else {
-        const uint32_t i = local * 64u + lane;
-        if (i < job.width) {
-          const float* rr = job.rays + 6ull * i;
-          const V3f o{rr[0], rr[1], rr[2]};
-          const V3f d{rr[3], rr[4], rr[5]};
-          const CastResult r = cast_ray(tv, o, d);
-          store_result(tv, job.out, i, r, o);
-        }
+        idx = local * 64u + lane;
+        valid = idx < job.width;
+        const float* rr = job.rays + 6ull * (valid ? idx : 0u);
+        o = V3f{rr[0], rr[1], rr[2]};
+        d = V3f{rr[3], rr[4], rr[5]};
+      }
+      if (valid) {
+        const CastResult r = cast_ray(tv, o, d);
+        store_result(tv, job.out, idx, r, o);
       }
     }
   }
 }
+
+template <int MIN_WAVES, int TPW, int REFILL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
+void batch_refill_kernel(const char* __restrict__ table, Counter* __restrict__ counters) {
+  const BatchHeader* hdr = header_of(table);
+  const BatchJob* jobs = jobs_of(table);
+  const TreeView* trees = trees_of(table);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t njobs = hdr->njobs;
+  const uint32_t total = hdr->total_tiles;
+  const uint32_t chunk = hdr->chunk_tiles;
+  const uint32_t home = blockIdx.x % kChunks;
+  for (uint32_t k = 0; k < kChunks; ++k) {
+    const uint32_t c = (home + k) % kChunks;
+    const uint32_t begin = c * chunk;
+    const uint32_t end = min(total, begin + chunk);
+    if (begin >= end) continue;
+    for (;;) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(&counters[c].next, static_cast<uint32_t>(TPW));
+      t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+      if (t >= end - begin) break;
+      const uint32_t r0 = begin + t;
+      const uint32_t r1 = min(end, r0 + static_cast<uint32_t>(TPW));
+      // the grabbed tiles, one job segment at a time (job and tree uniform)
+      for (uint32_t tile = r0; tile < r1;) {
+        const uint32_t j = __builtin_amdgcn_readfirstlane(find_job(jobs, njobs, tile));
+        const BatchJob& job = jobs[j];
+        const TreeView& tv = trees[job.tree];
+        const uint32_t seg_end = min(r1, job.tile_begin + job.tile_count);
+        const uint32_t a = tile - job.tile_begin, b = seg_end - job.tile_begin;
+        if (job.kind == 0) {
+          const RenderSource src{job.cam, job.width, job.height, job.tiles_x, job.first_tile,
+                                 job.tile_major};
+          refill_loop<REFILL>(tv, job.out, a * 64u, b * 64u, src);
+        } else {
+          const ArraySource src{job.rays, job.width};
+          refill_loop<REFILL>(tv, job.out, a * 64u, min(b * 64u, job.width), src);
+        }
+        tile = seg_end;
+      }
+    }
+  }
+}
+
+using BatchFn = void (*)(const char*, Counter*);
+
+struct BatchVariant {
+  const char* name;
+  BatchFn fn;
+};
+
+const BatchVariant kBatchVariants[] = {
+    {"classic-w1", batch_kernel<1>},
+    {"classic-w5", batch_kernel<5>},
+    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>},
+    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>},
+};
+constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+int g_batch_variant = 0;
 
 int hip_fail(hipError_t e, const char* what, std::string* err) {
   *err = std::string(what) + ": " + hipGetErrorString(e);
   return e == hipErrorOutOfMemory ? SVO_OUT_OF_DEVICE_MEMORY : SVO_HIP_ERROR;
 }
 
+size_t trees_offset_for(uint32_t njobs) {
+  return (kJobsOffset + njobs * sizeof(BatchJob) + 63) & ~size_t(63);
+}
+
 }  // namespace
 
 struct BatchContext {
   int device = -1;
+  int variant = 0;
   uint32_t max_jobs = 0;
-  size_t table_bytes = 0;  // header + trees + jobs
+  size_t table_bytes = 0;  // capacity: header + jobs + trees
   void* host = nullptr;    // pinned staging of the table
   void* dev = nullptr;     // device table
   Counter* counters = nullptr;
   uint32_t grid = 0;
-  hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
+  hipGraphNode_t copy_node = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t done = nullptr;
   bool pending = false;
   uint64_t last_tiles = 0;
+  // kernel arguments of the graph's kernel nodes
+  const char* arg_table = nullptr;
+  Counter* arg_counters = nullptr;
 };
 
 namespace {
 
-size_t trees_offset() { return 256; }
-size_t jobs_offset(uint32_t max_jobs) {
-  return (trees_offset() + max_jobs * sizeof(TreeView) + 255) & ~size_t(255);
-}
-
-int capture(BatchContext* b, std::string* err) {
-  hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture", err);
-  const char* base = static_cast<const char*>(b->dev);
-  (void)hipMemcpyAsync(b->dev, b->host, b->table_bytes, hipMemcpyHostToDevice, b->capture_stream);
-  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, b->capture_stream, b->counters);
-  hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), 0, b->capture_stream,
-                     reinterpret_cast<const BatchHeader*>(base),
-                     reinterpret_cast<const TreeView*>(base + trees_offset()),
-                     reinterpret_cast<const BatchJob*>(base + jobs_offset(b->max_jobs)),
-                     b->counters);
-  e = hipStreamEndCapture(b->capture_stream, &b->graph);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", err);
+int build_graph(BatchContext* b, std::string* err) {
+  hipError_t e = hipGraphCreate(&b->graph, 0);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphCreate", err);
+  e = hipGraphAddMemcpyNode1D(&b->copy_node, b->graph, nullptr, 0, b->dev, b->host,
+                              b->table_bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphAddMemcpyNode1D", err);
+  b->arg_table = static_cast<const char*>(b->dev);
+  b->arg_counters = b->counters;
+  hipGraphNode_t reset_node = nullptr, kernel_node = nullptr;
+  void* reset_args[] = {&b->arg_counters};
+  hipKernelNodeParams rp{};
+  rp.func = reinterpret_cast<void*>(reset_counters_kernel);
+  rp.gridDim = dim3(1);
+  rp.blockDim = dim3(64);
+  rp.sharedMemBytes = 0;
+  rp.kernelParams = reset_args;
+  rp.extra = nullptr;
+  e = hipGraphAddKernelNode(&reset_node, b->graph, &b->copy_node, 1, &rp);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphAddKernelNode(reset)", err);
+  void* args[] = {&b->arg_table, &b->arg_counters};
+  hipKernelNodeParams kp{};
+  kp.func = reinterpret_cast<void*>(kBatchVariants[b->variant].fn);
+  kp.gridDim = dim3(b->grid);
+  kp.blockDim = dim3(256);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  e = hipGraphAddKernelNode(&kernel_node, b->graph, &reset_node, 1, &kp);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphAddKernelNode(batch)", err);
   e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", err);
   return SVO_OK;
 }
 
 }  // namespace
+
+int batch_variant_count() { return kBatchVariantCount; }
+const char* batch_variant_name(int i) {
+  return (i >= 0 && i < kBatchVariantCount) ? kBatchVariants[i].name : nullptr;
+}
+bool batch_variant_select(int i) {
+  if (i < 0 || i >= kBatchVariantCount) return false;
+  g_batch_variant = i;
+  return true;
+}
+int batch_variant_current() { return g_batch_variant; }
 
 int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
   *out = nullptr;
@@ -176,7 +295,8 @@ int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
   }
   BatchContext* b = new BatchContext();
   b->max_jobs = max_jobs;
-  b->table_bytes = jobs_offset(max_jobs) + max_jobs * sizeof(BatchJob);
+  b->variant = g_batch_variant;
+  b->table_bytes = trees_offset_for(max_jobs) + max_jobs * sizeof(TreeView);
   hipError_t e = hipGetDevice(&b->device);
   int cus = 0;
   if (e == hipSuccess)
@@ -185,7 +305,6 @@ int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
   if (e == hipSuccess) e = hipMalloc(&b->dev, b->table_bytes);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void**>(&b->counters), kChunks * sizeof(Counter));
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
   if (e != hipSuccess) {
     const int st = hip_fail(e, "batch_create", err);
@@ -193,9 +312,9 @@ int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
     return st;
   }
   std::memset(b->host, 0, b->table_bytes);
-  // four 256-lane workgroups per CU keep every SIMD at the kernel's occupancy
+  // four 256-lane workgroups per CU keep every SIMD at the kernels' occupancy
   b->grid = static_cast<uint32_t>(std::max(cus, 1)) * 4u;
-  const int st = capture(b, err);
+  const int st = build_graph(b, err);
   if (st != SVO_OK) {
     batch_destroy(b);
     return st;
@@ -211,7 +330,6 @@ void batch_destroy(BatchContext* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->done) (void)hipEventDestroy(b->done);
-  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   if (b->counters) (void)hipFree(b->counters);
   if (b->dev) (void)hipFree(b->dev);
   if (b->host) (void)hipHostFree(b->host);
@@ -219,6 +337,7 @@ void batch_destroy(BatchContext* b) {
 }
 
 uint64_t batch_tiles(const BatchContext* b) { return b ? b->last_tiles : 0; }
+int batch_context_variant(const BatchContext* b) { return b ? b->variant : -1; }
 
 int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream, std::string* err) {
   if (n > b->max_jobs) {
@@ -233,8 +352,9 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
   }
   char* base = static_cast<char*>(b->host);
   BatchHeader* hdr = reinterpret_cast<BatchHeader*>(base);
-  TreeView* trees = reinterpret_cast<TreeView*>(base + trees_offset());
-  BatchJob* out = reinterpret_cast<BatchJob*>(base + jobs_offset(b->max_jobs));
+  BatchJob* out = reinterpret_cast<BatchJob*>(base + kJobsOffset);
+  const size_t trees_offset = trees_offset_for(n);
+  TreeView* trees = reinterpret_cast<TreeView*>(base + trees_offset);
   std::unordered_map<const void*, uint32_t> tree_index;
   uint64_t tiles = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -260,7 +380,16 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     j.height = in.kind == 0 ? in.height : 1u;
     if (in.kind == 0) {
       j.tiles_x = (in.width + 7) / 8;
-      j.tile_count = j.tiles_x * ((in.height + 7) / 8);
+      const uint64_t image_tiles = static_cast<uint64_t>(j.tiles_x) * ((in.height + 7) / 8);
+      const uint64_t limit = in.tile_limit ? in.tile_limit : image_tiles - std::min<uint64_t>(
+                                                                  in.first_tile, image_tiles);
+      if (image_tiles >= (1ull << 26) || in.first_tile + limit > image_tiles) {
+        *err = "render job tile range outside the image, or image above 2^32 pixels";
+        return SVO_INVALID_ARGUMENT;
+      }
+      j.tile_count = static_cast<uint32_t>(limit);
+      j.first_tile = in.first_tile;
+      j.tile_major = in.tile_major ? 1u : 0u;
       j.cam = in.cam;
     } else {
       if (in.width && !in.rays) {
@@ -274,17 +403,22 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     j.tile_begin = static_cast<uint32_t>(tiles);
     j.out = to_ptrs(in.out);
     tiles += j.tile_count;
-    if (tiles >= (1ull << 31)) {
-      *err = "too many tiles in one batch";
+    if (tiles >= (1ull << 26)) {
+      *err = "too many tiles in one batch (at most 2^26 tiles of 64 rays)";
       return SVO_INVALID_ARGUMENT;
     }
   }
   hdr->njobs = n;
   hdr->total_tiles = static_cast<uint32_t>(tiles);
   hdr->chunk_tiles = static_cast<uint32_t>((tiles + kChunks - 1) / kChunks);
+  hdr->trees_offset = static_cast<uint32_t>(trees_offset);
   b->last_tiles = tiles;
+  const size_t used = trees_offset + tree_index.size() * sizeof(TreeView);
+  hipError_t e = hipGraphExecMemcpyNodeSetParams1D(b->exec, b->copy_node, b->dev, b->host, used,
+                                                   hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphExecMemcpyNodeSetParams1D", err);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipError_t e = hipGraphLaunch(b->exec, s);
+  e = hipGraphLaunch(b->exec, s);
   if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch", err);
   e = hipEventRecord(b->done, s);
   if (e != hipSuccess) return hip_fail(e, "hipEventRecord", err);

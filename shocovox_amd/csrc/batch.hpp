@@ -19,11 +19,26 @@ struct BatchJobIn {
   Camera cam;
   const float* rays;
   svo_hits out;
+  // Render chunks: the job covers tiles [first_tile, first_tile + tile_limit)
+  // of the image (tile_limit 0 = to the end). With tile_major set, outputs
+  // are stored tile after tile (64 slots per tile, pixel (x&7) + 8*(y&7)
+  // inside a tile) from out[0], instead of at y * width + x.
+  uint32_t first_tile = 0;
+  uint32_t tile_limit = 0;
+  uint32_t tile_major = 0;
 };
 
 int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err);
 void batch_destroy(BatchContext* b);
 int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream, std::string* err);
 uint64_t batch_tiles(const BatchContext* b);
+
+// Kernel variants of batched launches; a context uses the variant selected
+// when it was created.
+int batch_variant_count();
+const char* batch_variant_name(int i);
+bool batch_variant_select(int i);
+int batch_variant_current();
+int batch_context_variant(const BatchContext* b);
 
 }  // namespace svo

@@ -8,25 +8,12 @@
 #include "batch.hpp"
 #include "device.hpp"
 #include "flatten.hpp"
+#include "handles.hpp"
 #include "octree.hpp"
 
 namespace svo {
 Status build_scene(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t seed, Octree* out);
 }
-
-struct svo_octree {
-  svo::Octree tree;
-};
-
-struct svo_flat {
-  svo::LayoutMirror mirror;
-};
-
-struct svo_tree {
-  svo::DeviceTree* dev = nullptr;
-  svo::LayoutMirror mirror;  // host copy of the layout, for in-place edits
-  bool has_mirror = false;
-};
 
 namespace {
 
@@ -36,6 +23,12 @@ int fail(int status, const std::string& msg) {
   g_error = msg;
   return status;
 }
+
+}  // namespace
+
+void svo::set_last_error(const std::string& msg) { g_error = msg; }
+
+namespace {
 
 int from_status(svo::Status s) {
   switch (s) {
@@ -309,6 +302,33 @@ int svo_tree_from_octree(svo_octree* tree, svo_tree** out) {
   return SVO_OK;
 }
 
+int svo_tree_admit(svo_octree* tree, uint64_t budget_bytes, uint64_t reserve_bytes,
+                   svo_tree** out) {
+  if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  svo::LayoutMirror mirror;
+  (void)tree->tree.take_dirty();
+  mirror.build(tree->tree);
+  svo_tree_desc d;
+  fill_desc(mirror.flat(), &d);
+  const uint64_t need = svo::device_tree_bytes_for(d);
+  uint64_t avail = budget_bytes;
+  if (avail == 0) {
+    const int st = svo_device_memory(&avail, nullptr);
+    if (st != SVO_OK) return st;
+  }
+  if (need + reserve_bytes > avail)
+    return fail(SVO_OUT_OF_DEVICE_MEMORY,
+                "tree needs " + std::to_string(need) + " device bytes plus a reserve of " +
+                    std::to_string(reserve_bytes) + ", " + std::to_string(avail) + " available");
+  const int st = svo_tree_create(&d, out);
+  if (st != SVO_OK) return st;
+  (*out)->mirror = std::move(mirror);
+  (*out)->mirror.mark_synced();
+  (*out)->has_mirror = true;
+  return SVO_OK;
+}
+
 int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream) {
   if (!dtree || !tree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   std::string err;
@@ -371,6 +391,18 @@ int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32
   return SVO_OK;
 }
 
+int svo_render_rgba8(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
+                     uint32_t convention, uint32_t* out, void* stream) {
+  if (!dtree || !vp) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::string err;
+  const int st = svo::device_render_rgba8(
+      *dtree->dev, svo::V3f{vp->origin[0], vp->origin[1], vp->origin[2]},
+      svo::V3f{vp->direction[0], vp->direction[1], vp->direction[2]}, vp->frustum[0],
+      vp->frustum[1], vp->fov, w, h, convention, out, stream, &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
 int svo_batch_create(uint32_t max_jobs, svo_batch** out) {
   if (!out) return fail(SVO_INVALID_ARGUMENT, "out is NULL");
   svo::BatchContext* b = nullptr;
@@ -415,6 +447,15 @@ int svo_batch_run(svo_batch* batch, const svo_job* jobs, uint32_t n, void* strea
 uint64_t svo_batch_tiles(const svo_batch* batch) {
   return svo::batch_tiles(reinterpret_cast<const svo::BatchContext*>(batch));
 }
+
+int svo_batch_variant_count(void) { return svo::batch_variant_count(); }
+const char* svo_batch_variant_name(int variant) { return svo::batch_variant_name(variant); }
+int svo_select_batch_variant(int variant) {
+  if (!svo::batch_variant_select(variant))
+    return fail(SVO_INVALID_ARGUMENT, "no such batch variant");
+  return SVO_OK;
+}
+int svo_current_batch_variant(void) { return svo::batch_variant_current(); }
 
 int svo_ray_occupancy_lut(uint64_t* out512) {
   if (!out512) return fail(SVO_INVALID_ARGUMENT, "out is NULL");

@@ -13,6 +13,7 @@
 #include "device.hpp"
 #include "device_common.hpp"
 #include "flatten.hpp"
+#include "refill.hpp"
 
 namespace svo {
 
@@ -54,12 +55,105 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
+// Primary-ray render with lane refill (refill.hpp). Each 64-lane wave owns
+// TPW consecutive 8x8 tiles of the image; finished lanes take the next
+// pixels of that range while the others keep stepping.
+template <int MIN_WAVES, int TPW, int REFILL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
+void render_refill_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) {
+  const uint32_t tiles_x = (w + 7u) / 8u;
+  const uint32_t total_tiles = tiles_x * ((h + 7u) / 8u);
+  const uint32_t first_tile = (blockIdx.x * 4u + (threadIdx.x >> 6)) * TPW;
+  if (first_tile >= total_tiles) return;
+  const RenderSource src{cam, w, h, tiles_x, 0u, 0u};
+  refill_loop<REFILL>(t, out, first_tile * 64u, min(total_tiles, first_tile + TPW) * 64u, src);
+}
+
+// Shaded render to RGBA8 (r in the low byte), output row 0 at the top.
+// CONV 0 follows the reference's CPU example (examples/cpu_render.rs:78-137):
+//   ray from the eye through the glass point bottom_left + right * x *
+//   pixel_w + up * y * pixel_h (pixel corners), pixel (x, y) written to row
+//   h - 1 - y; a hit is shaded albedo * (1 - (normal . L / 2 + 0.5)) with
+//   L = normalize(0, -1, 1), each channel (c as f32 * s) as u8; a miss is
+//   grey 128.
+// CONV 1 follows the WGSL update() kernel (viewport_render.wgsl:1028-1089):
+//   the ray starts at the glass point bottom_left + right * frustum.x *
+//   (x / W8) + up * frustum.y * (1 - row / H8), W8 and H8 being the image
+//   size rounded up to the 8x8 workgroup grid, and points away from the eye;
+//   a hit is albedo * (normal . (-0.5, 0.5, -0.5) / 2 + 0.5), a miss grey
+//   0.25; channels are stored as unorm8 floor(clamp(v, 0, 1) * 255 + 0.5).
+struct ShadeParams {
+  V3f origin, bottom_left, right, up, light;
+  float pixel_w, pixel_h, glass_w, glass_h;
+  uint32_t w8, h8;
+};
+
+__device__ inline uint32_t sat_u8(float v) {
+  // Rust `as u8`: NaN -> 0, saturating, truncating toward zero
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 255.0f) return 255u;
+  return static_cast<uint32_t>(v);
+}
+
+__device__ inline uint32_t unorm8(float v) {
+  v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+  return static_cast<uint32_t>(floorf(v * 255.0f + 0.5f));
+}
+
+template <int CONV>
+__global__ __launch_bounds__(256) void shade_kernel(TreeView t, ShadeParams sp, uint32_t w,
+                                                    uint32_t h, uint32_t* __restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t x = (blockIdx.x * 2u + (wave & 1u)) * 8u + (lane & 7u);
+  const uint32_t row = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane >> 3);
+  if (x >= w || row >= h) return;
+  V3f o, d;
+  if (CONV == 0) {
+    const uint32_t y = h - 1u - row;
+    const V3f glass = (sp.bottom_left + (sp.right * static_cast<float>(x)) * sp.pixel_w) +
+                      (sp.up * static_cast<float>(y)) * sp.pixel_h;
+    o = sp.origin;
+    d = normalized(glass - sp.origin);
+  } else {
+    const V3f glass =
+        (sp.bottom_left +
+         (sp.right * sp.glass_w) * (static_cast<float>(x) / static_cast<float>(sp.w8))) +
+        (sp.up * sp.glass_h) * (1.0f - static_cast<float>(row) / static_cast<float>(sp.h8));
+    o = glass;
+    d = normalized(glass - sp.origin);
+  }
+  const CastResult r = cast_ray(t, o, d);
+  uint32_t rgba;
+  if (r.hit) {
+    const uint32_t ci = pix_color_index(r.pix);
+    const uint32_t c = (ci < NONE16 && ci < t.color_count) ? t.colors[ci] : 0u;
+    const uint32_t cr = c >> 24, cg = (c >> 16) & 0xFFu, cb = (c >> 8) & 0xFFu;
+    const float dot = (r.normal.x * sp.light.x + r.normal.y * sp.light.y) + r.normal.z * sp.light.z;
+    if (CONV == 0) {
+      const float strength = 1.0f - (dot / 2.0f + 0.5f);
+      rgba = sat_u8(static_cast<float>(cr) * strength) |
+             (sat_u8(static_cast<float>(cg) * strength) << 8) |
+             (sat_u8(static_cast<float>(cb) * strength) << 16) | 0xFF000000u;
+    } else {
+      const float k = dot / 2.0f + 0.5f;
+      rgba = unorm8((static_cast<float>(cr) / 255.0f) * k) |
+             (unorm8((static_cast<float>(cg) / 255.0f) * k) << 8) |
+             (unorm8((static_cast<float>(cb) / 255.0f) * k) << 16) | 0xFF000000u;
+    }
+  } else {
+    rgba = CONV == 0 ? 0xFF808080u : (unorm8(0.25f) * 0x010101u) | 0xFF000000u;
+  }
+  out[row * w + x] = rgba;
+}
+
 using RenderFn = void (*)(TreeView, Camera, uint32_t, uint32_t, HitPtrs);
 
 struct RenderVariant {
   const char* name;
   RenderFn fn;
   uint32_t tile_w, tile_h;
+  uint32_t tiles_per_wave;  // refill kernels: 8x8 tiles per wave (0: one tile per wave)
 };
 
 const RenderVariant kRenderVariants[] = {
@@ -72,6 +166,10 @@ const RenderVariant kRenderVariants[] = {
     {"w1-glut-16x4", render_kernel<1, false, 16, 4>, 16, 4},
     {"w1-glut-4x16", render_kernel<1, false, 4, 16>, 4, 16},
     {"w6-slut-16x4", render_kernel<6, true, 16, 4>, 16, 4},
+    // lane refill measured 3.5x slower than one tile per wave on coherent
+    // 1080p primary rays (lanes drift apart and lose the shared node lines);
+    // kept as a variant for divergent workloads
+    {"r1-t4-f16", render_refill_kernel<1, 4, 16>, 8, 8, 4},
 };
 constexpr int kRenderVariantCount = sizeof(kRenderVariants) / sizeof(kRenderVariants[0]);
 int g_render_variant = 0;
@@ -205,6 +303,21 @@ uint32_t timing_collect(float* out_ms, uint32_t cap) {
   return n;
 }
 
+namespace {
+// Capacity an upload allocates for `need` elements: a quarter of headroom so
+// that edits which add nodes or bricks are patched in place.
+uint64_t grown_capacity(uint64_t need) { return need + need / 4 + 1; }
+}  // namespace
+
+uint64_t device_tree_bytes_for(const svo_tree_desc& d) {
+  const uint64_t n3 = static_cast<uint64_t>(d.brick_dim) * d.brick_dim * d.brick_dim;
+  const uint64_t words = occ_words_per_brick(d.brick_dim);
+  const uint64_t bricks = grown_capacity(d.brick_count ? d.brick_count : 1);
+  return grown_capacity(d.node_count) * sizeof(NodeRec) + bricks * (n3 + words) * sizeof(uint32_t) +
+         grown_capacity(d.color_count ? d.color_count : 1) * sizeof(uint32_t) +
+         64 * 8 * sizeof(uint64_t);
+}
+
 int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std::string* err) {
   if (!validate_desc(d, err)) return SVO_INVALID_ARGUMENT;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -214,7 +327,7 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   const uint64_t words = occ_words_per_brick(d.brick_dim);
   auto grow = [&](void** p, uint64_t* cap, uint64_t need, uint64_t elem) -> int {
     if (need <= *cap && *p) return SVO_OK;
-    uint64_t nc = need + need / 4 + 1;
+    uint64_t nc = grown_capacity(need);
     void* np = nullptr;
     hipError_t ge = hipMalloc(&np, nc * elem);
     if (ge != hipSuccess) return hip_fail(ge, "hipMalloc", err);
@@ -453,10 +566,65 @@ int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
   const RenderVariant& v = kRenderVariants[g_render_variant];
-  const dim3 grid((w + 2 * v.tile_w - 1) / (2 * v.tile_w), (h + 2 * v.tile_h - 1) / (2 * v.tile_h));
+  dim3 grid((w + 2 * v.tile_w - 1) / (2 * v.tile_w), (h + 2 * v.tile_h - 1) / (2 * v.tile_h));
+  if (v.tiles_per_wave) {
+    // four waves per workgroup, each over tiles_per_wave consecutive 8x8 tiles
+    const uint64_t tiles = static_cast<uint64_t>((w + 7) / 8) * ((h + 7) / 8);
+    if (tiles * 64 >= (1ull << 32)) {
+      *err = "image too large for the refill kernels";
+      return SVO_INVALID_ARGUMENT;
+    }
+    const uint64_t waves = (tiles + v.tiles_per_wave - 1) / v.tiles_per_wave;
+    grid = dim3(static_cast<uint32_t>((waves + 3) / 4));
+  }
   hipLaunchKernelGGL(v.fn, grid, dim3(256), 0, s, t.view, cam, w, h, to_ptrs(out));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "render_kernel launch", err);
+  timer.done();
+  return SVO_OK;
+}
+
+int device_render_rgba8(const DeviceTree& t, V3f origin, V3f dir, float glass_w, float glass_h,
+                        float fov, uint32_t w, uint32_t h, uint32_t convention, uint32_t* out,
+                        void* stream, std::string* err) {
+  if (convention > 1) {
+    *err = "convention must be 0 (CPU example) or 1 (WGSL viewport)";
+    return SVO_INVALID_ARGUMENT;
+  }
+  if (w == 0 || h == 0) return SVO_OK;
+  if (!out) {
+    *err = "output is NULL";
+    return SVO_INVALID_ARGUMENT;
+  }
+  const V3f up{0.0f, 1.0f, 0.0f};
+  ShadeParams sp;
+  sp.origin = origin;
+  sp.up = up;
+  sp.right = normalized(cross(up, dir));
+  sp.glass_w = glass_w;
+  sp.glass_h = glass_h;
+  sp.pixel_w = glass_w / static_cast<float>(w);
+  sp.pixel_h = glass_h / static_cast<float>(h);
+  sp.w8 = (w + 7u) / 8u * 8u;
+  sp.h8 = (h + 7u) / 8u * 8u;
+  if (convention == 0) {
+    // viewport_bottom_left of cpu_render.rs:92-94, light of :97
+    sp.bottom_left = ((origin + dir * fov) - up * (glass_h / 2.0f)) - sp.right * (glass_w / 2.0f);
+    sp.light = normalized(V3f{0.0f, -1.0f, 1.0f});
+  } else {
+    // bottom left of viewport_render.wgsl:1033-1043, light of :1060
+    sp.bottom_left = ((origin + dir * fov) - sp.right * (glass_w / 2.0f)) - up * (glass_h / 2.0f);
+    sp.light = V3f{-0.5f, 0.5f, -0.5f};
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  LaunchTimer timer(s);
+  const dim3 grid((w + 15) / 16, (h + 15) / 16);
+  if (convention == 0)
+    hipLaunchKernelGGL(shade_kernel<0>, grid, dim3(256), 0, s, t.view, sp, w, h, out);
+  else
+    hipLaunchKernelGGL(shade_kernel<1>, grid, dim3(256), 0, s, t.view, sp, w, h, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "shade_kernel launch", err);
   timer.done();
   return SVO_OK;
 }

@@ -37,6 +37,9 @@ int device_tree_create(const svo_tree_desc& desc, DeviceTree** out, std::string*
 // needed; ordered on `stream`.
 int device_tree_upload(DeviceTree* t, const svo_tree_desc& desc, void* stream, std::string* err);
 void device_tree_destroy(DeviceTree* t);
+// Device bytes a fresh device_tree_create of `desc` allocates (capacities
+// include the growth headroom of uploads).
+uint64_t device_tree_bytes_for(const svo_tree_desc& desc);
 // Writes the given records and brick slots of a layout into an existing tree
 // whose capacities already cover `desc` (the layout's full description);
 // returns SVO_OUT_OF_DEVICE_MEMORY-free status or asks for a full upload by
@@ -49,6 +52,10 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
                 void* stream, std::string* err);
 int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
                   const svo_hits& out, void* stream, std::string* err);
+// Shaded RGBA8 render (conventions described at shade_kernel).
+int device_render_rgba8(const DeviceTree& t, V3f origin, V3f dir, float glass_w, float glass_h,
+                        float fov, uint32_t w, uint32_t h, uint32_t convention, uint32_t* out,
+                        void* stream, std::string* err);
 
 // Launch variants of the render kernel (occupancy hint, LUT placement, tile
 // shape); the process-wide selection applies to later svo_render calls.

@@ -1,0 +1,29 @@
+// Definitions of the opaque handles of include/svo.h, shared by the C ABI
+// glue and the serving loop.
+#pragma once
+
+#include <string>
+
+#include "../../include/svo.h"
+#include "device.hpp"
+#include "flatten.hpp"
+#include "octree.hpp"
+
+struct svo_octree {
+  svo::Octree tree;
+};
+
+struct svo_flat {
+  svo::LayoutMirror mirror;
+};
+
+struct svo_tree {
+  svo::DeviceTree* dev = nullptr;
+  svo::LayoutMirror mirror;  // host copy of the layout, for in-place edits
+  bool has_mirror = false;
+};
+
+namespace svo {
+// Sets the message svo_last_error() returns on the calling thread.
+void set_last_error(const std::string& msg);
+}  // namespace svo

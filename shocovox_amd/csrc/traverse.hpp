@@ -71,8 +71,10 @@ SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
 }
 
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
-SVO_HD inline bool probe_brick(const TreeView& t, uint32_t form, uint32_t word, Cube bb, V3f d,
-                               V3f sig, V3f scale, V3f* p, CastResult* r) {
+SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t, uint32_t form,
+                                                              uint32_t word, Cube bb, V3f d,
+                                                              V3f sig, V3f scale, V3f* p,
+                                                              CastResult* r) {
   if (form == FORM_EMPTY) return false;
   if (form == FORM_SOLID) {
     r->pix = word;
@@ -128,7 +130,9 @@ SVO_HD inline bool probe_brick(const TreeView& t, uint32_t form, uint32_t word, 
 // get_by_ray_at_lod(ray, f32::MAX) with MIPs disabled. `lut` is the
 // RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the caller staged it.
 template <class LutT>
-SVO_HD inline CastResult cast_ray(const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d) {
+SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t,
+                                                                const LutT* __restrict__ lut,
+                                                                V3f o, V3f d) {
   CastResult r;
   r.hit = 0;
   r.pix = EMPTY_MARKER;
@@ -260,8 +264,180 @@ SVO_HD inline CastResult cast_ray(const TreeView& t, const LutT* __restrict__ lu
   return r;
 }
 
-SVO_HD inline CastResult cast_ray(const TreeView& t, V3f o, V3f d) {
+SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t, V3f o,
+                                                                V3f d) {
   return cast_ray(t, t.ray_lut, o, d);
+}
+
+// The same traversal as cast_ray in resumable form: ray_begin sets a ray up,
+// each ray_step runs one iteration of the stack loop (and the restart from
+// the root when the stack empties). A kernel can then hand a finished lane a
+// new ray while the other lanes of its wave keep stepping. The operation
+// sequence per ray is identical to cast_ray's. Only the traversal state and
+// the flags live across steps; the result is written when the ray finishes.
+struct RayState {
+  V3f d, scale, p;
+  Cube bounds;
+  uint32_t head, count, target, steps, flags;
+  // the ring stack's four slots as separate scalars: a dynamically indexed
+  // array inside the state would be kept in scratch memory
+  uint32_t stk0, stk1, stk2, stk3;
+  SVO_HD uint32_t stk_get(uint32_t i) const {
+    return i == 0u ? stk0 : (i == 1u ? stk1 : (i == 2u ? stk2 : stk3));
+  }
+  SVO_HD void stk_set(uint32_t i, uint32_t v) {
+    stk0 = i == 0u ? v : stk0;
+    stk1 = i == 1u ? v : stk1;
+    stk2 = i == 2u ? v : stk2;
+    stk3 = i == 3u ? v : stk3;
+  }
+};
+
+// A finished ray that hit nothing, as cast_ray returns it.
+SVO_HD inline CastResult miss_result(uint32_t flags) {
+  CastResult r;
+  r.hit = 0;
+  r.pix = EMPTY_MARKER;
+  r.flags = flags;
+  r.impact = V3f{0.0f, 0.0f, 0.0f};
+  r.normal = V3f{0.0f, 0.0f, 0.0f};
+  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
+  return r;
+}
+
+// false when the ray misses the tree outright (its result is miss_result(0))
+SVO_HD __attribute__((always_inline)) inline bool ray_begin(const TreeView& t, V3f o, V3f d,
+                                                           RayState& s) {
+  s.d = d;
+  s.scale = dda_scale_factors(d);
+  const Cube root{v3u(0.0f), t.size};
+  float tt = 0.0f;
+  const int ir = intersect_ray(root, o, d, &tt);
+  if (ir == 0) return false;
+  s.p = o + d * (ir == 2 ? tt : 0.0f);
+  s.target = hash_region(s.p - root.min, root.size / 2.0f);
+  if (s.target == OOB_OCTANT) return false;
+  s.bounds = root;
+  s.stk0 = 0u;
+  s.stk1 = 0u;
+  s.stk2 = 0u;
+  s.stk3 = 0u;
+  s.head = 1u;
+  s.count = 1u;
+  s.steps = 0u;
+  s.flags = 0u;
+  return true;
+}
+
+// true when the ray is finished; r then holds its result
+template <class LutT>
+SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
+                                                          const LutT* __restrict__ lut,
+                                                          RayState& s, CastResult& r) {
+  const float lo = 0.00001f;
+  const float hi = 4.0f - 0.00001f;
+  if (++s.steps > MAX_TRAVERSAL_STEPS) {
+    r = miss_result(s.flags | FLAG_ITERATION_GUARD);
+    return true;
+  }
+  // cheap functions of the direction, recomputed rather than kept per lane
+  const V3f sig{rsignum(s.d.x), rsignum(s.d.y), rsignum(s.d.z)};
+  const uint32_t dir_idx = hash_direction(s.d);
+  const NodeRec& n = t.nodes[s.stk_get(s.head)];
+  const uint64_t occ = n.occ;
+  const uint32_t meta = n.meta;
+  const uint32_t kind = node_kind(meta);
+  bool backtrack = kind == KIND_UNIFORM_LEAF;
+  if (s.target != OOB_OCTANT && kind != KIND_INTERNAL && kind != KIND_NOTHING) {
+    const bool uniform = kind == KIND_UNIFORM_LEAF;
+    const uint32_t slot = uniform ? 0u : s.target;
+    CastResult hr = miss_result(0u);
+    if (probe_brick(t, brick_form(meta, slot), n.child[slot],
+                    uniform ? s.bounds : child_bounds_for(s.bounds, s.target), s.d, sig,
+                    s.scale, &s.p, &hr)) {
+      r = hr;
+      r.hit = 1;
+      r.flags = s.flags | hr.flags;
+      return true;
+    }
+    s.flags |= hr.flags;
+  }
+  const float rcp_size = pow2_rcp(s.bounds.size);
+  const V3f rel = (s.p - s.bounds.min) * 4.0f;
+  V3f bp{div_by(rel.x, s.bounds.size, rcp_size), div_by(rel.y, s.bounds.size, rcp_size),
+         div_by(rel.z, s.bounds.size, rcp_size)};
+  bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
+  uint32_t flat = bitmap_index(bp, &s.flags);
+  if (backtrack || s.target == OOB_OCTANT || occ == 0ull ||
+      (occ & lut[flat * 8u + dir_idx]) == 0ull) {
+    // POP
+    s.head = (s.head + 3u) & 3u;
+    s.count -= 1u;
+    if (s.count != 0u) {
+      const V3f center = s.bounds.min + v3u(s.bounds.size / 2.0f);
+      const float s2 = s.bounds.size * 2.0f;
+      const float rcp_s2 = pow2_rcp(s2);
+      const V3f pmin{sub_fmod(s.bounds.min.x, s2, rcp_s2), sub_fmod(s.bounds.min.y, s2, rcp_s2),
+                     sub_fmod(s.bounds.min.z, s2, rcp_s2)};
+      const uint32_t from = hash_region(center - pmin, s.bounds.size);
+      const V3f st = dda_step(s.d, sig, &s.p, s.bounds, s.scale);
+      s.target = step_octant(from, st);
+      s.bounds.size *= 2.0f;
+      s.bounds.min = pmin;
+    }
+  } else {
+    Cube tb = child_bounds_for(s.bounds, s.target);
+    uint32_t tck = kind == KIND_INTERNAL ? n.child[s.target] : INVALID_INDEX;
+    if (tck != INVALID_INDEX && (occ & octant_bitmap_mask(s.target)) != 0ull) {
+      // PUSH
+      s.bounds = tb;
+      s.target = hash_region(s.p - tb.min, tb.size / 2.0f);
+      s.head = (s.head + 1u) & 3u;
+      s.count = s.count < 4u ? s.count + 1u : 4u;
+      s.stk_set(s.head, tck);
+    } else {
+      // ADVANCE
+      for (;;) {
+        if (++s.steps > MAX_TRAVERSAL_STEPS) {
+          r = miss_result(s.flags | FLAG_ITERATION_GUARD);
+          return true;
+        }
+        const V3f st = dda_step(s.d, sig, &s.p, tb, s.scale);
+        s.target = step_octant(s.target, st);
+        if (s.target != OOB_OCTANT) {
+          tb = child_bounds_for(s.bounds, s.target);
+          tck = kind == KIND_INTERNAL ? n.child[s.target] : INVALID_INDEX;
+          const V3f sd = st * 4.0f;
+          bp = bp + V3f{div_by(sd.x, s.bounds.size, rcp_size),
+                        div_by(sd.y, s.bounds.size, rcp_size),
+                        div_by(sd.z, s.bounds.size, rcp_size)};
+          flat = bitmap_index(bp, &s.flags);
+        }
+        if (s.target == OOB_OCTANT ||
+            (tck != INVALID_INDEX && (occ & octant_bitmap_mask(s.target)) != 0ull &&
+             (lut[flat * 8u + dir_idx] & occ) != 0ull) ||
+            (kind == KIND_LEAF && brick_form(meta, s.target) != FORM_EMPTY))
+          break;
+      }
+    }
+  }
+  if (s.count == 0u) {
+    // restart from the root
+    s.p = s.p + s.d * 0.1f;
+    s.target = (s.p.x < t.size && s.p.y < t.size && s.p.z < t.size && s.p.x > 0.0f &&
+                s.p.y > 0.0f && s.p.z > 0.0f)
+                   ? hash_region(s.p, t.size / 2.0f)
+                   : OOB_OCTANT;
+    if (s.target == OOB_OCTANT) {
+      r = miss_result(s.flags);
+      return true;
+    }
+    s.bounds = Cube{v3u(0.0f), t.size};
+    s.head = (s.head + 1u) & 3u;
+    s.count = 1u;
+    s.stk_set(s.head, 0u);
+  }
+  return false;
 }
 
 // Voxel coordinate of a hit: floor(impact) clamped into the hit cell.

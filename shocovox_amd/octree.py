@@ -361,11 +361,29 @@ class DeviceTree:
         v = vp.to_c()
         _check(N.lib().svo_render(self._h, C.byref(v), w, h, C.byref(hs), _stream_ptr(stream)))
 
+    def render_rgba8(self, vp: Viewport, w: int, h: int, convention: int = 0, out=None,
+                     stream=None):
+        """Shaded image as an (h, w) int32 tensor of RGBA8 pixels (r in the low
+        byte), row 0 at the top. convention 0: the reference CPU example's
+        camera and shading; 1: the WGSL viewport kernel's."""
+        return _render_rgba8(self, vp, w, h, convention, out, stream)
+
     def render(self, vp: Viewport, w: int, h: int, fields=HIT_FIELDS, device="cuda",
                stream=None) -> dict:
         bufs = alloc_hits(w * h, device, fields)
         self.render_into(vp, w, h, bufs, stream)
         return bufs
+
+
+def _render_rgba8(tree: "DeviceTree", vp: "Viewport", w: int, h: int, convention: int,
+                  out=None, stream=None):
+    torch = _torch()
+    if out is None:
+        out = torch.empty((h, w), dtype=torch.int32, device="cuda")
+    v = vp.to_c()
+    _check(N.lib().svo_render_rgba8(tree._h, C.byref(v), w, h, convention, out.data_ptr(),
+                                    _stream_ptr(stream)))
+    return out
 
 
 class Batch:
@@ -401,6 +419,21 @@ class Batch:
     @property
     def tiles(self) -> int:
         return int(N.lib().svo_batch_tiles(self._h))
+
+
+def batch_variants() -> list:
+    """Names of the batched-launch kernel variants, in index order."""
+    L = N.lib()
+    return [L.svo_batch_variant_name(i).decode() for i in range(L.svo_batch_variant_count())]
+
+
+def select_batch_variant(index: int) -> None:
+    """Kernel variant of batch contexts created from now on."""
+    _check(N.lib().svo_select_batch_variant(index))
+
+
+def current_batch_variant() -> int:
+    return int(N.lib().svo_current_batch_variant())
 
 
 def render_variants() -> list:

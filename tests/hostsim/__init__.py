@@ -29,7 +29,7 @@ def lib():
     if _lib is None:
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
         L = C.CDLL(str(LIB))
-        L.hs_cast.argtypes = [C.POINTER(_Tree), C.c_void_p, C.c_uint64, C.c_void_p]
+        L.hs_cast.argtypes = [C.POINTER(_Tree), C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
         L.hs_viewport_rays.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         _lib = L
     return _lib
@@ -42,7 +42,9 @@ def ray_lut() -> np.ndarray:
     return out
 
 
-def cast(flat_arrays: dict, rays: np.ndarray) -> np.ndarray:
+def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False) -> np.ndarray:
+    """The product traversal on the host: cast_ray, or with resumable=True
+    its ray_begin / ray_step form."""
     a = flat_arrays
     keep = [np.ascontiguousarray(a["nodes"]), np.ascontiguousarray(a["brick_pix"]),
             np.ascontiguousarray(a["brick_occ"]), np.ascontiguousarray(a["colors"]), ray_lut()]
@@ -51,7 +53,7 @@ def cast(flat_arrays: dict, rays: np.ndarray) -> np.ndarray:
               (dim ** 3 + 31) // 32, len(a["colors"]))
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
     out = np.zeros(len(rays), HIT_DTYPE)
-    lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data)
+    lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, int(resumable))
     return out
 
 

@@ -28,13 +28,26 @@ struct hs_hit {
   float normal[3];
 };
 
-void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out) {
+// mode 0: cast_ray; mode 1: the resumable form (ray_begin, then ray_step
+// until the ray finishes), as the refill kernels run it.
+void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode) {
   svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut, t->size, t->dim,
                   t->occ_words, t->color_count, svo::pow2_rcp(static_cast<float>(t->dim))};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
     const svo::V3f o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
-    const svo::CastResult c = svo::cast_ray(v, o, d);
+    svo::CastResult c;
+    if (mode == 0) {
+      c = svo::cast_ray(v, o, d);
+    } else {
+      svo::RayState st;
+      if (svo::ray_begin(v, o, d, st)) {
+        while (!svo::ray_step(v, v.ray_lut, st, c)) {
+        }
+      } else {
+        c = svo::miss_result(0u);
+      }
+    }
     hs_hit& h = out[i];
     h.hit = c.hit;
     h.flags = c.flags;

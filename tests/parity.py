@@ -49,3 +49,27 @@ def assert_parity(expected: np.ndarray, got: dict, label: str = "") -> dict:
     bad = {k: v for k, v in res.items() if k != "rays" and v}
     assert not bad, f"{label}: mismatching rays per field {bad} of {res['rays']}"
     return res
+
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def result_digest(h) -> int:
+    """Digest of a row-major render result, as the serving loop computes it
+    on the device: the wrapping sum over pixels i of a splitmix64 chain over
+    (i, hit, albedo, voxel x, y, z, distance bits)."""
+    n = len(h["hit"])
+    a = _mix64(np.arange(n, dtype=np.uint64) + _G)
+    vox = np.asarray(h["voxel"]).reshape(n, 3)
+    dist = np.ascontiguousarray(h["distance"], dtype=np.float32).view(np.uint32)
+    for f in (h["hit"], h["albedo"], vox[:, 0], vox[:, 1], vox[:, 2], dist):
+        a = _mix64(a ^ np.asarray(f).astype(np.uint32).astype(np.uint64))
+    return int(a.sum(dtype=np.uint64))

@@ -225,12 +225,18 @@ def test_in_place_edits_on_device(gpu_device):
         assert_parity(exp, gpu_to_numpy(bufs), f"edit round {rnd}")
 
 
-def test_batched_mixed_jobs_and_replays(gpu_device):
+@pytest.mark.parametrize("variant", range(4))
+def test_batched_mixed_jobs_and_replays(gpu_device, variant):
     """Many trees, uneven viewports and ray arrays in one persistent launch;
-    the captured graph is replayed with different job sets (with eager work in
-    between) and every job of every replay equals the oracle."""
+    the graph is replayed with different job sets (with eager work in
+    between) and every job of every replay equals the oracle - for every
+    batch kernel variant."""
     import torch
-    from shocovox_amd import Batch, Octree, alloc_hits
+    from shocovox_amd import (Batch, Octree, alloc_hits, batch_variants, current_batch_variant,
+                              select_batch_variant)
+    names = batch_variants()
+    assert len(names) == 4
+    _say(f"batch variant {names[variant]}")
     rng = np.random.default_rng(23)
     trees = [W.build_scene(W.PLAIN, seed=s) for s in (1, 2)]
     small = Octree.new(64, 4)
@@ -242,7 +248,12 @@ def test_batched_mixed_jobs_and_replays(gpu_device):
     trees.append(small)
     dtrees = [DeviceTree(t) for t in trees]
     oracles = [_oracle(t) for t in trees]
-    batch = Batch(16)
+    prev = current_batch_variant()
+    select_batch_variant(variant)
+    try:
+        batch = Batch(16)
+    finally:
+        select_batch_variant(prev)
     for replay in range(3):
         jobs, checks = [], []
         for k in range(int(rng.integers(3, 9))):

@@ -241,3 +241,38 @@ def test_incremental_layout_matches_fresh_tree(seed):
         for p in rng.integers(0, size, size=(300, 3)):
             p = tuple(int(v) for v in p)
             assert flat.get(p) == tree.get(p), p
+
+
+def test_resumable_traversal_equals_cast_ray():
+    """ray_begin / ray_step (the refill kernels' form) give the same result,
+    bit for bit, as cast_ray and the oracle on the plain workload, a caves
+    sample, the KAT trees and incoherent rays into small random trees."""
+    cases = []
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    cases.append((tree, oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov,
+                                             cfg.width, cfg.height)))
+    caves = W.build_scene(W.CAVES, seed=1)
+    vp = W.viewports(W.CAVES)[3]
+    cases.append((caves, oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov,
+                                              W.CAVES.width, W.CAVES.height)[::53]))
+    for name, (builder, r, _) in fixed_kats().items():
+        cases.append((builder(), r[None, :]))
+    rng = np.random.default_rng(77)
+    for size, dim in [(16, 2), (64, 4)]:
+        t = Octree.new(size, dim)
+        xyz = rng.integers(0, size, size=(size * size, 3)).astype(np.uint32)
+        ents = np.zeros((len(xyz), 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = 0xFF00FFFF
+        t.insert_many(xyz, ents)
+        cases.append((t, W.incoherent_rays(size, 3000, seed=size)))
+    for t, rays in cases:
+        a = t.flatten().arrays()
+        base = hostsim.cast(a, rays)
+        res = hostsim.cast(a, rays, resumable=True)
+        assert base.tobytes() == res.tobytes()
+        exp = oracle.OracleTree(t.export_model()).cast(rays)
+        r = compare(exp, {k: res[k] for k in res.dtype.names})
+        assert all(v == 0 for k, v in r.items() if k != "rays"), r
