@@ -34,24 +34,46 @@ __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __re
 // wave covers a TW x TH pixel tile (TW * TH = 64). MIN_WAVES asks the
 // compiler for at least that many waves per SIMD (it caps VGPRs to fit);
 // LDS_LUT stages the 4 KiB RAY_TO_NODE_OCCUPANCY_BITMASK_LUT in LDS.
-template <int MIN_WAVES, bool LDS_LUT, int TW, int TH>
+// NODES > 0 stages the first NODES node records (the top levels of the
+// breadth-first numbering) in LDS. XCD remaps the workgroup order so that
+// each of the 8 XCDs (which receive workgroups round-robin in dispatch
+// order) renders one contiguous band of the image, keeping that band's nodes
+// and bricks in its own L2.
+template <int MIN_WAVES, bool LDS_LUT, int TW, int TH, int NODES = 0, bool XCD = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) {
   static_assert(TW * TH == 64, "a wave covers 64 pixels");
   __shared__ uint64_t s_lut[LDS_LUT ? 512 : 1];
+  __shared__ NodeRec s_nodes[NODES > 0 ? NODES : 1];
   const uint64_t* lut = t.ray_lut;
   if constexpr (LDS_LUT) {
     for (uint32_t i = threadIdx.x; i < 512; i += 256) s_lut[i] = t.ray_lut[i];
-    __syncthreads();
     lut = s_lut;
+  }
+  uint32_t ntop = 0;
+  if constexpr (NODES > 0) {
+    ntop = min(static_cast<uint32_t>(NODES), t.node_count);
+    const uint4* src = reinterpret_cast<const uint4*>(t.nodes);
+    uint4* dst = reinterpret_cast<uint4*>(s_nodes);
+    for (uint32_t i = threadIdx.x; i < ntop * 4u; i += 256) dst[i] = src[i];
+  }
+  if constexpr (LDS_LUT || NODES > 0) __syncthreads();
+  uint32_t bx = blockIdx.x, by = blockIdx.y;
+  if constexpr (XCD) {
+    const uint32_t groups = gridDim.x * gridDim.y;
+    const uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t xcd = b % 8u, k = b / 8u, q = groups / 8u, rem = groups % 8u;
+    const uint32_t logical = xcd * q + min(xcd, rem) + k;
+    bx = logical % gridDim.x;
+    by = logical / gridDim.x;
   }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t x = (blockIdx.x * 2u + (wave & 1u)) * TW + (lane % TW);
-  const uint32_t y = (blockIdx.y * 2u + (wave >> 1)) * TH + (lane / TW);
+  const uint32_t x = (bx * 2u + (wave & 1u)) * TW + (lane % TW);
+  const uint32_t y = (by * 2u + (wave >> 1)) * TH + (lane / TW);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
-  const CastResult res = cast_ray(t, lut, cam.origin, d);
+  const CastResult res = cast_ray(t, lut, cam.origin, d, s_nodes, ntop);
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
@@ -166,6 +188,12 @@ const RenderVariant kRenderVariants[] = {
     {"w1-glut-16x4", render_kernel<1, false, 16, 4>, 16, 4},
     {"w1-glut-4x16", render_kernel<1, false, 4, 16>, 4, 16},
     {"w6-slut-16x4", render_kernel<6, true, 16, 4>, 16, 4},
+    {"w5-glut-8x8-xcd", render_kernel<5, false, 8, 8, 0, true>, 8, 8},
+    {"w1-glut-8x8-xcd", render_kernel<1, false, 8, 8, 0, true>, 8, 8},
+    {"w5-glut-8x8-n73", render_kernel<5, false, 8, 8, 73>, 8, 8},
+    {"w1-glut-8x8-n73", render_kernel<1, false, 8, 8, 73>, 8, 8},
+    {"w1-glut-8x8-n585", render_kernel<1, false, 8, 8, 585>, 8, 8},
+    {"w5-glut-8x8-n73-xcd", render_kernel<5, false, 8, 8, 73, true>, 8, 8},
     // lane refill measured 3.5x slower than one tile per wave on coherent
     // 1080p primary rays (lanes drift apart and lose the shared node lines);
     // kept as a variant for divergent workloads
@@ -401,6 +429,7 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   t->view.occ_words = static_cast<uint32_t>(words);
   t->view.color_count = d.color_count;
   t->view.inv_dim = pow2_rcp(static_cast<float>(d.brick_dim));
+  t->view.node_count = d.node_count;
   return SVO_OK;
 }
 
@@ -526,6 +555,7 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   t->brick_count = d.brick_count;
   t->color_count = d.color_count;
   t->view.color_count = d.color_count;
+  t->view.node_count = d.node_count;
   return SVO_OK;
 }
 

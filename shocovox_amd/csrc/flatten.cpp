@@ -1,9 +1,11 @@
 // Host octree -> device layout.  See layout.hpp for the record format.
 #include "flatten.hpp"
 
-#include <cmath>
 #include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <cstring>
+#include <thread>
 #include <unordered_map>
 
 namespace svo {
@@ -17,6 +19,29 @@ inline bool bound_contains(Cube b, V3f p) {
 
 }  // namespace
 
+// Runs f(i) for i in [0, n) on up to 16 threads.
+template <class F>
+void parallel_range(size_t n, F&& f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned nt = static_cast<unsigned>(
+      std::min<size_t>(std::max<size_t>(1, n / 64), std::max(1u, std::min(hw ? hw : 1u, 16u))));
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t)
+    pool.emplace_back([&]() {
+      for (;;) {
+        const size_t i = next.fetch_add(1);
+        if (i >= n) break;
+        f(i);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
 FlatTree flatten(const Octree& tree) {
   FlatTree f;
   f.octree_size = tree.size();
@@ -29,10 +54,11 @@ FlatTree flatten(const Octree& tree) {
   const auto& kids = tree.node_children();
 
   // Breadth-first order over keys; `index_of` maps pool key -> record index.
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
   std::vector<uint32_t> order;
-  std::unordered_map<uint32_t, uint32_t> index_of;
+  std::vector<uint32_t> index_of(pool.len(), kNone);
   order.push_back(Octree::ROOT_NODE_KEY);
-  index_of.emplace(Octree::ROOT_NODE_KEY, 0);
+  index_of[Octree::ROOT_NODE_KEY] = 0;
   size_t level_begin = 0;
   while (level_begin < order.size()) {
     const size_t level_stop = order.size();
@@ -42,8 +68,8 @@ FlatTree flatten(const Octree& tree) {
       if (n.kind != NodeKind::Internal) continue;
       for (uint32_t o = 0; o < 8; ++o) {
         const uint32_t c = key < kids.size() ? kids[key].child(o) : EMPTY_MARKER;
-        if (!pool.key_is_valid(c) || index_of.count(c)) continue;
-        index_of.emplace(c, static_cast<uint32_t>(order.size()));
+        if (!pool.key_is_valid(c) || index_of[c] != kNone) continue;
+        index_of[c] = static_cast<uint32_t>(order.size());
         order.push_back(c);
       }
     }
@@ -51,16 +77,10 @@ FlatTree flatten(const Octree& tree) {
     level_begin = level_stop;
   }
 
+  // Records, with Parted bricks numbered in record order; the brick words
+  // are then copied and their occupancy sidecars built in parallel.
   f.nodes.resize(order.size());
-  auto add_brick = [&](const std::vector<uint32_t>& voxels) {
-    const uint32_t slot = static_cast<uint32_t>(f.brick_pix.size() / (n3 ? n3 : 1));
-    f.brick_pix.insert(f.brick_pix.end(), voxels.begin(), voxels.end());
-    const size_t base = f.brick_occ.size();
-    f.brick_occ.resize(base + words, 0u);
-    for (size_t i = 0; i < n3; ++i)
-      if (!tree.pix_points_to_empty(voxels[i])) f.brick_occ[base + i / 32] |= 1u << (i % 32);
-    return slot;
-  };
+  std::vector<const std::vector<uint32_t>*> parted;
   auto encode_brick = [&](const Brick& b, uint32_t octant, NodeRec& r) {
     uint32_t form = FORM_EMPTY;
     uint32_t word = 0;
@@ -69,12 +89,12 @@ FlatTree flatten(const Octree& tree) {
       word = b.solid;
     } else if (b.kind == BrickKind::Parted) {
       form = FORM_PARTED;
-      word = add_brick(b.parted);
+      word = static_cast<uint32_t>(parted.size());
+      parted.push_back(&b.parted);
     }
     r.meta |= form << (8u + 2u * octant);
     r.child[octant] = word;
   };
-
   for (size_t i = 0; i < order.size(); ++i) {
     const uint32_t key = order[i];
     const NodeContent& n = pool.get(key);
@@ -90,7 +110,7 @@ FlatTree flatten(const Octree& tree) {
         r.meta = KIND_INTERNAL;
         for (uint32_t o = 0; o < 8; ++o) {
           const uint32_t c = key < kids.size() ? kids[key].child(o) : EMPTY_MARKER;
-          if (pool.key_is_valid(c)) r.child[o] = index_of.at(c);
+          if (pool.key_is_valid(c)) r.child[o] = index_of[c];
         }
         break;
       case NodeKind::Leaf:
@@ -103,6 +123,16 @@ FlatTree flatten(const Octree& tree) {
         break;
     }
   }
+  f.brick_pix.resize(parted.size() * n3);
+  f.brick_occ.assign(parted.size() * words, 0u);
+  parallel_range(parted.size(), [&](size_t slot) {
+    const std::vector<uint32_t>& voxels = *parted[slot];
+    uint32_t* pix = f.brick_pix.data() + slot * n3;
+    uint32_t* occ = f.brick_occ.data() + slot * words;
+    std::memcpy(pix, voxels.data(), n3 * sizeof(uint32_t));
+    for (size_t i = 0; i < n3; ++i)
+      if (!tree.pix_points_to_empty(voxels[i])) occ[i / 32] |= 1u << (i % 32);
+  });
   return f;
 }
 

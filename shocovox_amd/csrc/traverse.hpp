@@ -38,6 +38,7 @@ struct TreeView {
   uint32_t occ_words;
   uint32_t color_count;
   float inv_dim;  // pow2_rcp(dim): exact since the brick dimension is a power of two
+  uint32_t node_count;
 };
 
 struct CastResult {
@@ -129,10 +130,15 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
 
 // get_by_ray_at_lod(ray, f32::MAX) with MIPs disabled. `lut` is the
 // RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the caller staged it.
+// `top` optionally holds a copy of the first `ntop` node records (the top
+// levels, as records are numbered breadth-first), e.g. staged in LDS; those
+// records are read from it instead of t.nodes.
 template <class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t,
                                                                 const LutT* __restrict__ lut,
-                                                                V3f o, V3f d) {
+                                                                V3f o, V3f d,
+                                                                const NodeRec* top = nullptr,
+                                                                uint32_t ntop = 0u) {
   CastResult r;
   r.hit = 0;
   r.pix = EMPTY_MARKER;
@@ -175,7 +181,8 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
         r.flags |= FLAG_ITERATION_GUARD;
         return r;
       }
-      const NodeRec& n = t.nodes[stk[head]];
+      const uint32_t key = stk[head];
+      const NodeRec& n = (key < ntop ? top : t.nodes)[key];
       const uint64_t occ = n.occ;
       const uint32_t meta = n.meta;
       const uint32_t kind = node_kind(meta);

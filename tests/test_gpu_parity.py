@@ -286,3 +286,53 @@ def test_batched_mixed_jobs_and_replays(gpu_device, variant):
     # an empty batch is valid
     batch.run([])
     torch.cuda.synchronize()
+
+
+def test_spheres_2048_full_size(gpu_device):
+    """The 2048^3 / dim 32 spheres workload at full size: 3840x2160 primary
+    rays and 8M incoherent rays on the GPU. Every ray: no guard flag, every
+    hit's voxel is filled under the host octree's get() and carries the hit's
+    albedo, distance = |impact - origin|. A strided sample of each ray set
+    equals the oracle bit for bit."""
+    import torch
+    cfg = W.SPHERES
+    _say("spheres 2048^3 build")
+    tree = W.build_scene(cfg, seed=1)
+    d = DeviceTree(tree)
+    vp = W.viewports(cfg)[0]
+    _say(f"spheres render {cfg.width}x{cfg.height}")
+    fields = ("hit", "pix", "albedo", "voxel", "distance", "impact", "flags")
+    bufs = d.render(vp, cfg.width, cfg.height, fields=fields)
+    torch.cuda.synchronize()
+    prim = gpu_to_numpy(bufs)
+    del bufs
+    inc_rays = W.incoherent_rays(cfg.size, 8_000_000, seed=7)
+    _say("spheres incoherent 8000000")
+    t = torch.from_numpy(inc_rays).to(gpu_device)
+    bufs = d.cast(t, fields=fields)
+    torch.cuda.synchronize()
+    inc = gpu_to_numpy(bufs)
+    del bufs, t
+    prim_rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width,
+                                     cfg.height)
+    for name, got, rays in (("primary", prim, prim_rays), ("incoherent", inc, inc_rays)):
+        assert (got["flags"] == 0).all(), name
+        hit = got["hit"] == 1
+        _say(f"{name}: {int(hit.sum())} hits of {len(hit)}")
+        assert hit.sum() > 0.05 * len(hit)
+        vox = got["voxel"].reshape(-1, 3)[hit]
+        ent = tree.get_many(vox)
+        assert (ent[:, 0] != 0).all(), f"{name}: a hit voxel is empty"
+        assert (ent[:, 1] == got["albedo"][hit]).all(), f"{name}: albedo differs from get()"
+        dist = np.linalg.norm(got["impact"].reshape(-1, 3)[hit].astype(np.float64)
+                              - rays[hit, :3].astype(np.float64), axis=1)
+        assert np.allclose(dist, got["distance"][hit], rtol=1e-5, atol=1e-3), name
+    _say("spheres oracle sample")
+    otree = _oracle(tree)
+    for name, got, rays, step in (("primary", prim, prim_rays, 83),
+                                  ("incoherent", inc, inc_rays, 79)):
+        sample = np.ascontiguousarray(rays[::step])
+        exp = otree.cast(sample, threads=16)
+        sub = {k: v.reshape(len(rays), -1)[::step].reshape((-1,) + v.shape[1:])
+               for k, v in got.items()}
+        assert_parity(exp, sub, f"spheres {name} sample")
