@@ -118,6 +118,16 @@ typedef struct svo_model_dump {
 
 int svo_octree_export(const svo_octree* tree, svo_model_sizes* sizes, svo_model_dump* dump);
 
+/* Persisted trees in the reference's format: the bencode of Octree<u32>
+ * that Octree::save / Octree::to_bytes write (octree/mod.rs:136-168,
+ * convert/bytecode.rs). MIP maps in a loaded file are dropped (the traversal
+ * here runs with MIPs off); saved files carry empty MIPs and a disabled MIP
+ * strategy. to_bytes: call with out == NULL to learn the size. */
+int svo_octree_to_bytes(const svo_octree* tree, uint8_t* out, size_t cap, size_t* len);
+int svo_octree_from_bytes(const uint8_t* data, size_t len, svo_octree** out);
+int svo_octree_save(const svo_octree* tree, const char* path);
+int svo_octree_load(const char* path, svo_octree** out);
+
 /* ---------------------------------------------------------------------- */
 /* Synthetic scenes (the benchmark workloads), built with the host octree.  */
 /* ---------------------------------------------------------------------- */
@@ -386,6 +396,12 @@ int svo_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
  * returns SVO_OUT_OF_DEVICE_MEMORY without touching the device. */
 int svo_tree_admit(svo_octree* tree, uint64_t budget_bytes, uint64_t reserve_bytes,
                    svo_tree** out);
+
+/* Octree::load (octree/mod.rs:161-168) straight into a device tree: parses
+ * the file, flattens and uploads it. If host_out is not NULL it receives the
+ * host octree (for later edits with svo_tree_update); otherwise the host
+ * copy is released once the tree is resident. */
+int svo_tree_load(const char* path, svo_octree** host_out, svo_tree** out);
 
 /* Launch variants of the render kernel (no counterpart in the reference,
  * whose WGSL dispatch is fixed at 8x8 workgroups): occupancy hint, placement

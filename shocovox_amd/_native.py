@@ -116,6 +116,11 @@ _SIGS = {
     "svo_octree_get_many": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "svo_octree_export": (C.c_int, [_P, C.POINTER(ModelSizes), C.POINTER(ModelDump)]),
     "svo_scene_build": (C.c_int, [_U32, _U32, _U32, C.c_uint64, C.POINTER(_P)]),
+    "svo_octree_to_bytes": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "svo_octree_from_bytes": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "svo_octree_save": (C.c_int, [_P, C.c_char_p]),
+    "svo_octree_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "svo_tree_load": (C.c_int, [C.c_char_p, C.POINTER(_P), C.POINTER(_P)]),
     "svo_octree_flatten": (C.c_int, [_P, C.POINTER(_P)]),
     "svo_flat_desc": (C.c_int, [_P, C.POINTER(TreeDesc)]),
     "svo_flat_sync": (C.c_int, [_P, _P]),

@@ -1,11 +1,14 @@
 // extern "C" boundary of libsvo (include/svo.h).
 #include <cstring>
+#include <fstream>
+#include <iterator>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "../../include/svo.h"
 #include "batch.hpp"
+#include "bytecode.hpp"
 #include "device.hpp"
 #include "flatten.hpp"
 #include "handles.hpp"
@@ -299,6 +302,66 @@ int svo_tree_from_octree(svo_octree* tree, svo_tree** out) {
   (*out)->mirror = std::move(mirror);
   (*out)->mirror.mark_synced();
   (*out)->has_mirror = true;
+  return SVO_OK;
+}
+
+int svo_octree_to_bytes(const svo_octree* tree, uint8_t* out, size_t cap, size_t* len) {
+  if (!tree || !len) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::string b;
+  svo::octree_to_bencode(tree->tree, &b);
+  *len = b.size();
+  if (out) {
+    if (cap < b.size()) return fail(SVO_INVALID_ARGUMENT, "output buffer too small");
+    std::memcpy(out, b.data(), b.size());
+  }
+  return SVO_OK;
+}
+
+int svo_octree_from_bytes(const uint8_t* data, size_t len, svo_octree** out) {
+  if (!out || (len && !data)) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  svo_octree* t = new (std::nothrow) svo_octree();
+  if (!t) return fail(SVO_INVALID_ARGUMENT, "out of host memory");
+  std::string err;
+  if (!svo::octree_from_bencode(data, len, &t->tree, &err)) {
+    delete t;
+    return fail(SVO_INVALID_STRUCTURE, "not a serialized octree: " + err);
+  }
+  *out = t;
+  return SVO_OK;
+}
+
+int svo_octree_save(const svo_octree* tree, const char* path) {
+  if (!tree || !path) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::string b;
+  svo::octree_to_bencode(tree->tree, &b);
+  std::ofstream f(path, std::ios::binary | std::ios::trunc);
+  if (!f || !f.write(b.data(), static_cast<std::streamsize>(b.size())))
+    return fail(SVO_INVALID_ARGUMENT, std::string("cannot write ") + path);
+  return SVO_OK;
+}
+
+int svo_octree_load(const char* path, svo_octree** out) {
+  if (!path || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(SVO_INVALID_ARGUMENT, std::string("cannot read ") + path);
+  const std::string b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  return svo_octree_from_bytes(reinterpret_cast<const uint8_t*>(b.data()), b.size(), out);
+}
+
+int svo_tree_load(const char* path, svo_octree** host_out, svo_tree** out) {
+  if (!out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (host_out) *host_out = nullptr;
+  svo_octree* t = nullptr;
+  int st = svo_octree_load(path, &t);
+  if (st != SVO_OK) return st;
+  st = svo_tree_from_octree(t, out);
+  if (st != SVO_OK || !host_out) {
+    delete t;
+    return st;
+  }
+  *host_out = t;
   return SVO_OK;
 }
 

@@ -265,6 +265,24 @@ uint32_t Octree::add_to_palette(const Entry& e) {
   return EMPTY_MARKER;
 }
 
+// Octree::decode_bencode_object (convert/bytecode.rs:580-660): palettes and
+// their reverse maps (built in index order, so a later duplicate wins)
+void Octree::assign_model(std::vector<NodeContent> items, std::vector<uint8_t> reserved,
+                          size_t first_available, std::vector<NodeChildren> children,
+                          std::vector<uint32_t> colors, std::vector<uint32_t> data,
+                          bool simplify) {
+  nodes_.assign(std::move(items), std::move(reserved), first_available);
+  node_children_ = std::move(children);
+  color_palette_ = std::move(colors);
+  data_palette_ = std::move(data);
+  color_index_.clear();
+  data_index_.clear();
+  for (uint32_t i = 0; i < color_palette_.size(); ++i) color_index_[color_palette_[i]] = i;
+  for (uint32_t i = 0; i < data_palette_.size(); ++i) data_index_[data_palette_[i]] = i;
+  auto_simplify = simplify;
+  ++edit_epoch_;
+}
+
 // pix_points_to_empty (node.rs:405-427)
 bool Octree::pix_points_to_empty(uint32_t pix) const {
   const uint32_t c = pix_color_index(pix), d = pix_data_index(pix);

@@ -134,6 +134,17 @@ class NodePool {
     items_.reserve(n);
     reserved_.reserve(n);
   }
+  size_t first_available() const { return first_available_; }
+  // Replaces the pool's contents (a deserialized pool); every key is dirty.
+  void assign(std::vector<NodeContent> items, std::vector<uint8_t> reserved,
+              size_t first_available) {
+    items_ = std::move(items);
+    reserved_ = std::move(reserved);
+    first_available_ = first_available;
+    dirty_flag_.clear();
+    dirty_.clear();
+    for (uint32_t k = 0; k < items_.size(); ++k) touch(k);
+  }
   void touch(uint32_t key) {
     if (key >= dirty_flag_.size()) dirty_flag_.resize(static_cast<size_t>(key) + 1, 0);
     if (!dirty_flag_[key]) {
@@ -199,6 +210,12 @@ class Octree {
   const std::vector<uint32_t>& color_palette() const { return color_palette_; }
   const std::vector<uint32_t>& data_palette() const { return data_palette_; }
   uint64_t stored_occupied_bits(uint32_t key) const;
+  // Installs a deserialized model (node pool, children, palettes) in place of
+  // this tree's; palette lookup maps are rebuilt as Octree::from_bencode does
+  // (a later duplicate palette entry wins).
+  void assign_model(std::vector<NodeContent> items, std::vector<uint8_t> reserved,
+                    size_t first_available, std::vector<NodeChildren> children,
+                    std::vector<uint32_t> colors, std::vector<uint32_t> data, bool simplify);
   void store_occupied_bits(uint32_t key, uint64_t bits);
   bool simplify(uint32_t key);
   // Counts every edit applied since the last call (for incremental re-upload).

@@ -103,6 +103,33 @@ class Octree:
         return cls(size, brick_dim)
 
     @classmethod
+    def from_bytes(cls, data: bytes) -> "Octree":
+        """Octree::from_bytes (octree/mod.rs:143-147): the reference's bencode."""
+        h = C.c_void_p()
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data) if data else None
+        _check(N.lib().svo_octree_from_bytes(buf, len(data), C.byref(h)))
+        return cls(0, 0, _handle=h.value)
+
+    @classmethod
+    def load(cls, path: str) -> "Octree":
+        """Octree::load (octree/mod.rs:161-168)."""
+        h = C.c_void_p()
+        _check(N.lib().svo_octree_load(str(path).encode(), C.byref(h)))
+        return cls(0, 0, _handle=h.value)
+
+    def to_bytes(self) -> bytes:
+        """Octree::to_bytes (octree/mod.rs:136-140)."""
+        n = C.c_size_t()
+        _check(N.lib().svo_octree_to_bytes(self._h, None, 0, C.byref(n)))
+        buf = (C.c_uint8 * max(1, n.value))()
+        _check(N.lib().svo_octree_to_bytes(self._h, buf, n.value, C.byref(n)))
+        return bytes(buf[:n.value])
+
+    def save(self, path: str) -> None:
+        """Octree::save (octree/mod.rs:151-157)."""
+        _check(N.lib().svo_octree_save(self._h, str(path).encode()))
+
+    @classmethod
     def scene(cls, kind: int, size: int, brick_dim: int, seed: int) -> "Octree":
         """Builds one of the seeded benchmark scenes (see include/svo.h)."""
         h = C.c_void_p()
@@ -313,6 +340,17 @@ def _stream_ptr(stream) -> int | None:
 
 class DeviceTree:
     """A tree resident in HBM; answers get_by_ray for batches of rays."""
+
+    @classmethod
+    def load(cls, path: str, keep_host: bool = False):
+        """A saved octree file straight into device memory (svo_tree_load).
+        With keep_host, returns (DeviceTree, Octree) so that edits can follow."""
+        h, t = C.c_void_p(), C.c_void_p()
+        _check(N.lib().svo_tree_load(str(path).encode(), C.byref(t) if keep_host else None,
+                                     C.byref(h)))
+        d = cls.__new__(cls)
+        d._h = h.value
+        return (d, Octree(0, 0, _handle=t.value)) if keep_host else d
 
     def __init__(self, octree: Octree | None = None, desc: N.TreeDesc | None = None):
         lib = N.lib()

@@ -336,3 +336,28 @@ def test_spheres_2048_full_size(gpu_device):
         sub = {k: v.reshape(len(rays), -1)[::step].reshape((-1,) + v.shape[1:])
                for k, v in got.items()}
         assert_parity(exp, sub, f"spheres {name} sample")
+
+
+def test_saved_tree_loaded_straight_to_device(gpu_device, tmp_path):
+    """Octree::save, then svo_tree_load into device memory: renders equal the
+    oracle on the original tree; with the host copy kept, an edit followed by
+    svo_tree_update equals the oracle on the edited tree."""
+    import torch
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=5)
+    path = tmp_path / "plain.bencode"
+    tree.save(path)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    _say("loaded tree render")
+    d = DeviceTree.load(path)
+    got = gpu_to_numpy(d.render(vp, cfg.width, cfg.height))
+    torch.cuda.synchronize()
+    assert_parity(_oracle(tree).cast(rays), got, "loaded tree")
+    d2, host = DeviceTree.load(path, keep_host=True)
+    host.clear_at_lod((64, 64, 64), 32)
+    d2.update(host)
+    _say("loaded tree edited")
+    got = gpu_to_numpy(d2.render(vp, cfg.width, cfg.height))
+    torch.cuda.synchronize()
+    assert_parity(_oracle(host).cast(rays), got, "loaded tree after edit")
