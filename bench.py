@@ -255,6 +255,8 @@ def served_bench(args, capacity_mrays: float) -> dict:
                          "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
                          "render_p99_ms": round(r["render_p99_ms"], 3),
                          "edit_p99_ms": round(r["edit_p99_ms"], 3),
+                         "edit_phase_p99_ms": {k: round(r[f"edit_{k}_p99_ms"], 3)
+                                               for k in ("wait", "apply", "upload")},
                          "requests": int(r["requests"]), "edits": int(r["edits"]),
                          "launches": int(r["launches"]),
                          "jobs_per_launch": round(r["mean_jobs_per_launch"], 2),
@@ -296,7 +298,9 @@ def main() -> int:
     ap.add_argument("--served-max-size", type=int, default=512)
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--served-inflight", type=int, default=2)
-    ap.add_argument("--served-batch-rays", type=int, default=1 << 21)
+    # launches of 8M rays: a request of up to 4K completes in one launch, so a
+    # popular tree goes idle often enough for its edits to start promptly
+    ap.add_argument("--served-batch-rays", type=int, default=1 << 23)
     ap.add_argument("--served-loads", type=float, nargs="+", default=[0.3, 0.6, 0.9, 1.2],
                     help="offered load as fractions of the measured render capacity")
     ap.add_argument("--mode", default="render", choices=["render", "copy"],

@@ -365,6 +365,9 @@ typedef struct svo_serve_stats {
   double edit_p50_ms, edit_p99_ms;        /* edit requests only */
   double mean_jobs_per_launch;
   uint64_t edit_records, edit_bricks;     /* node records / brick slots uploaded by edits */
+  /* phases of edit requests: arrival to start on the worker (waiting for the
+   * tree to go idle), host octree edit, layout update and device upload */
+  double edit_wait_p99_ms, edit_apply_p99_ms, edit_upload_p99_ms;
 } svo_serve_stats;
 
 /* The request sequence of a serving session (deterministic in cfg and the

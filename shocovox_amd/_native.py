@@ -91,7 +91,9 @@ class ServeStats(C.Structure):
                                              "mean_ms", "max_ms", "render_p50_ms",
                                              "render_p99_ms", "edit_p50_ms", "edit_p99_ms",
                                              "mean_jobs_per_launch")]
-                + [("edit_records", C.c_uint64), ("edit_bricks", C.c_uint64)])
+                + [("edit_records", C.c_uint64), ("edit_bricks", C.c_uint64)]
+                + [(k, C.c_double) for k in ("edit_wait_p99_ms", "edit_apply_p99_ms",
+                                             "edit_upload_p99_ms")])
 
 
 _P = C.c_void_p

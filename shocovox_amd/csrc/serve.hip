@@ -276,12 +276,17 @@ struct Slot {
   bool busy = false;
 };
 
+struct EditTimes {
+  double start = 0, applied = 0, uploaded = 0;
+};
+
 struct EditWorker {
   std::thread th;
   std::mutex mu;
   std::condition_variable cv;
   std::deque<uint32_t> todo;
   std::vector<std::pair<uint32_t, double>> finished;
+  std::vector<EditTimes> times;  // per request of the plan (edits only)
   bool stop = false;
   int status = SVO_OK;
   std::string error;
@@ -335,14 +340,19 @@ void edit_worker_main(Session* s) {
     int st = SVO_HIP_ERROR;
     std::string err = "edit stream missing";
     uint32_t recs = 0, bricks = 0;
+    EditTimes et;
+    et.start = s->now();
     if (stream) {
       st = apply_edit(s->trees[q.tree]->tree, q);
+      et.applied = s->now();
       if (st == SVO_OK) st = svo_tree_update(s->dtrees[q.tree], s->trees[q.tree], stream);
       if (st == SVO_OK) (void)svo_tree_update_stats(s->dtrees[q.tree], &recs, &bricks);
       err = st == SVO_OK ? "" : svo_last_error();
     }
     const double t = s->now();
+    et.uploaded = t;
     std::lock_guard<std::mutex> g(w.mu);
+    w.times[r] = et;
     if (st != SVO_OK && w.status == SVO_OK) {
       w.status = st;
       w.error = "edit failed: " + err;
@@ -526,6 +536,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
 
   hipError_t e = hipEventCreate(&s.ref);
   if (e != hipSuccess) return hip_err(e, "hipEventCreate");
+  s.worker.times.assign(n, EditTimes{});
   s.worker.th = std::thread(edit_worker_main, &s);
   e = hipEventRecord(s.ref, nullptr);
   if (e == hipSuccess) e = hipEventSynchronize(s.ref);
@@ -758,6 +769,17 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
         launches ? static_cast<double>(chunks_launched) / static_cast<double>(launches) : 0.0;
     o.edit_records = s.worker.records;
     o.edit_bricks = s.worker.bricks;
+    std::vector<double> wait, apply, upload;
+    for (size_t i = 0; i < n; ++i) {
+      if (s.plan[i].kind != 1 || rejected[i]) continue;
+      const EditTimes& et = s.worker.times[i];
+      wait.push_back((et.start - s.plan[i].arrival_s) * 1e3);
+      apply.push_back((et.applied - et.start) * 1e3);
+      upload.push_back((et.uploaded - et.applied) * 1e3);
+    }
+    o.edit_wait_p99_ms = percentile(wait, 0.99);
+    o.edit_apply_p99_ms = percentile(apply, 0.99);
+    o.edit_upload_p99_ms = percentile(upload, 0.99);
   }
   return SVO_OK;
 }

@@ -55,6 +55,12 @@ SVO_HD inline float rsignum(float x) {
 }
 
 // `f as i32` (saturating, NaN -> 0)
+// `s as i32` for a DDA step component. dda_step returns, per axis, either
+// +0.0 or signum(direction), i.e. one of {-1, +1, +0, NaN}; on that domain
+// this equals f2i32(s) and f2i32(roundf(s)) (NaN -> 0) without the general
+// conversion's range checks.
+SVO_HD inline int32_t step_int(float s) { return s == 1.0f ? 1 : (s == -1.0f ? -1 : 0); }
+
 SVO_HD inline int32_t f2i32(float x) {
   if (x != x) return 0;
   if (x >= 2147483648.0f) return 2147483647;
@@ -124,8 +130,8 @@ SVO_HD inline int isignum(int v) { return (v > 0) - (v < 0); }
 
 // step_octant (raytracing/mod.rs:68-80): `(step.c as i32).signum()`
 SVO_HD inline uint32_t step_octant(uint32_t octant, V3f step) {
-  return step_octant_signs(octant, isignum(f2i32(step.x)), isignum(f2i32(step.y)),
-                           isignum(f2i32(step.z)));
+  return step_octant_signs(octant, isignum(step_int(step.x)), isignum(step_int(step.y)),
+                           isignum(step_int(step.z)));
 }
 
 // Cube{min_position, size}  (spatial/mod.rs:31-53)
@@ -186,7 +192,9 @@ SVO_HD inline V3f dda_step(V3f dir, V3f sig, V3f* p, Cube b, V3f scale) {
   const float dx = fabsf(snx * scale.x);
   const float dy = fabsf(sny * scale.y);
   const float dz = fabsf(snz * scale.z);
-  const float m = rmin(rmin(dx, dy), dz);
+  // operands are |.| or NaN, where f32::min and fminf (IEEE minNum) agree:
+  // NaN yields the other operand and no -0.0 can occur
+  const float m = fminf(fminf(dx, dy), dz);
   *p = *p + dir * m;
   return V3f{m == dx ? sig.x : 0.0f, m == dy ? sig.y : 0.0f, m == dz ? sig.z : 0.0f};
 }
@@ -205,6 +213,11 @@ SVO_HD inline float pow2_rcp(float s) {
 
 // x / s, through the exact reciprocal when s is a power of two.
 SVO_HD inline float div_by(float x, float s, float rcp) { return rcp != 0.0f ? x * rcp : x / s; }
+// div_by for the three components of a vector with one test of the divisor
+SVO_HD inline V3f div3_by(V3f v, float s, float rcp) {
+  if (rcp != 0.0f) return V3f{v.x * rcp, v.y * rcp, v.z * rcp};
+  return V3f{v.x / s, v.y / s, v.z / s};
+}
 
 // a - fmodf(a, b) (spatial/math/vector.rs:66-71 as used by the POP at
 // raytracing_on_cpu.rs:452-456). For b a power of two >= 1 and 0 < a < 2^24:

@@ -58,17 +58,20 @@ SVO_HD inline float rclamp(float x, float lo, float hi) {
 }
 
 // BITMAP_INDEX_LUT[floor(x) as usize][floor(y)][floor(z)] = x + 4y + 16z
+// The reference converts with `as usize`; a saturating u32 conversion gives
+// the same value for 0..3 and a value above 3 (flagged) for everything else,
+// without the 64-bit conversion sequence.
 SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
-  uint64_t x = f2usize(floorf(bp.x));
-  uint64_t y = f2usize(floorf(bp.y));
-  uint64_t z = f2usize(floorf(bp.z));
+  uint32_t x = f2u32(floorf(bp.x));
+  uint32_t y = f2u32(floorf(bp.y));
+  uint32_t z = f2u32(floorf(bp.z));
   if (x > 3 || y > 3 || z > 3) {
     *flags |= FLAG_BITMAP_RANGE;
     x = x > 3 ? 3 : x;
     y = y > 3 ? 3 : y;
     z = z > 3 ? 3 : z;
   }
-  return static_cast<uint32_t>(x + 4 * y + 16 * z);
+  return x + 4u * y + 16u * z;
 }
 
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
@@ -88,8 +91,7 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   const float fdim = static_cast<float>(t.dim);
   const float rcp_size = pow2_rcp(bb.size);
   const V3f rel = (*p - bb.min) * fdim;
-  const V3f pib{div_by(rel.x, bb.size, rcp_size), div_by(rel.y, bb.size, rcp_size),
-                div_by(rel.z, bb.size, rcp_size)};
+  const V3f pib = div3_by(rel, bb.size, rcp_size);
   auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
   int ix = clampi(f2i32(pib.x));
   int iy = clampi(f2i32(pib.y));
@@ -103,16 +105,14 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   V3f step{0.0f, 0.0f, 0.0f};
   for (int guard = 0; guard < 4 * dim + 16; ++guard) {
     if (ix < 0 || ix >= dim || iy < 0 || iy >= dim || iz < 0 || iz >= dim) return false;
-    flat += f2i32(step.x) * 1 + f2i32(step.y) * dim + f2i32(step.z) * (dim * dim);
+    flat += step_int(step.x) * 1 + step_int(step.y) * dim + step_int(step.z) * (dim * dim);
     if ((occ[static_cast<uint32_t>(flat) >> 5] >> (static_cast<uint32_t>(flat) & 31u)) & 1u) {
       const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
       r->pix = t.brick_pix[static_cast<uint64_t>(word) * n3 + static_cast<uint32_t>(flat)];
       const V3f scaled = V3f{static_cast<float>(ix), static_cast<float>(iy),
                              static_cast<float>(iz)} *
                          bb.size;
-      const Cube hb{bb.min + V3f{div_by(scaled.x, fdim, t.inv_dim), div_by(scaled.y, fdim, t.inv_dim),
-                                 div_by(scaled.z, fdim, t.inv_dim)},
-                    div_by(bb.size, fdim, t.inv_dim)};
+      const Cube hb{bb.min + div3_by(scaled, fdim, t.inv_dim), div_by(bb.size, fdim, t.inv_dim)};
       r->impact = *p;
       r->normal = cube_impact_normal(hb, *p);
       r->cell = hb;
@@ -120,9 +120,9 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
     }
     step = dda_step(d, sig, p, cb, scale);
     cb.min = cb.min + step * unit;
-    ix += f2i32(roundf(step.x));
-    iy += f2i32(roundf(step.y));
-    iz += f2i32(roundf(step.z));
+    ix += step_int(step.x);
+    iy += step_int(step.y);
+    iz += step_int(step.z);
   }
   r->flags |= FLAG_ITERATION_GUARD;
   return false;
@@ -204,8 +204,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
       }
       const float rcp_size = pow2_rcp(bounds.size);
       const V3f rel = (p - bounds.min) * 4.0f;
-      V3f bp{div_by(rel.x, bounds.size, rcp_size), div_by(rel.y, bounds.size, rcp_size),
-             div_by(rel.z, bounds.size, rcp_size)};
+      V3f bp = div3_by(rel, bounds.size, rcp_size);
       bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
       uint32_t flat = bitmap_index(bp, &r.flags);
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
@@ -249,8 +248,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
             tb = child_bounds_for(bounds, target);
             tck = kind == KIND_INTERNAL ? n.child[target] : INVALID_INDEX;
             const V3f sd = st * 4.0f;
-            bp = bp + V3f{div_by(sd.x, bounds.size, rcp_size), div_by(sd.y, bounds.size, rcp_size),
-                          div_by(sd.z, bounds.size, rcp_size)};
+            bp = bp + div3_by(sd, bounds.size, rcp_size);
             flat = bitmap_index(bp, &r.flags);
           }
           if (target == OOB_OCTANT ||
@@ -371,8 +369,7 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
   }
   const float rcp_size = pow2_rcp(s.bounds.size);
   const V3f rel = (s.p - s.bounds.min) * 4.0f;
-  V3f bp{div_by(rel.x, s.bounds.size, rcp_size), div_by(rel.y, s.bounds.size, rcp_size),
-         div_by(rel.z, s.bounds.size, rcp_size)};
+  V3f bp = div3_by(rel, s.bounds.size, rcp_size);
   bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
   uint32_t flat = bitmap_index(bp, &s.flags);
   if (backtrack || s.target == OOB_OCTANT || occ == 0ull ||
@@ -415,9 +412,7 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
           tb = child_bounds_for(s.bounds, s.target);
           tck = kind == KIND_INTERNAL ? n.child[s.target] : INVALID_INDEX;
           const V3f sd = st * 4.0f;
-          bp = bp + V3f{div_by(sd.x, s.bounds.size, rcp_size),
-                        div_by(sd.y, s.bounds.size, rcp_size),
-                        div_by(sd.z, s.bounds.size, rcp_size)};
+          bp = bp + div3_by(sd, s.bounds.size, rcp_size);
           flat = bitmap_index(bp, &s.flags);
         }
         if (s.target == OOB_OCTANT ||

@@ -81,6 +81,15 @@ class OctreeError(N.SvoError):
     pass
 
 
+def _release(fn: str, handle: int) -> None:
+    """Frees a library handle; quiet during interpreter shutdown, when module
+    globals may already be gone."""
+    try:
+        getattr(N.lib(), fn)(handle)
+    except Exception:  # noqa: BLE001 - nothing to report to at shutdown
+        pass
+
+
 def _check(status: int) -> None:
     if status != N.SVO_OK:
         msg = N.lib().svo_last_error()
@@ -139,8 +148,8 @@ class Octree:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.lib().svo_octree_free(h)
             self._h = None
+            _release("svo_octree_free", h)
 
     @property
     def handle(self) -> int:
@@ -245,8 +254,8 @@ class FlatTree:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.lib().svo_flat_free(h)
             self._h = None
+            _release("svo_flat_free", h)
 
     def desc(self) -> N.TreeDesc:
         d = N.TreeDesc()
@@ -366,8 +375,8 @@ class DeviceTree:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.lib().svo_tree_destroy(h)
             self._h = None
+            _release("svo_tree_destroy", h)
 
     @property
     def device_bytes(self) -> int:
@@ -438,8 +447,8 @@ class Batch:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            N.lib().svo_batch_destroy(h)
             self._h = None
+            _release("svo_batch_destroy", h)
 
     @staticmethod
     def render_job(tree: "DeviceTree", vp: "Viewport", w: int, h: int, bufs: dict) -> N.Job:

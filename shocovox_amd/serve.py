@@ -30,7 +30,7 @@ class ServeConfig:
     seed: int = 1
     mode: str = "batched"
     max_inflight: int = 2
-    max_batch_rays: int = 1 << 21
+    max_batch_rays: int = 1 << 23
     max_batch_jobs: int = 1024
     max_queue: int = 0
     edit_fraction: float = 0.05
