@@ -200,6 +200,12 @@ int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream);
 int svo_tree_update_stats(const svo_tree* dtree, uint32_t* records, uint32_t* slots);
 void svo_tree_destroy(svo_tree* dtree);
 /* Device bytes held by the tree. */
+/* Iteration bound of one ray on this tree (an addition of this library: the
+ * reference loop has no bound). A ray that exceeds it ends as a miss with
+ * the iteration-guard flag (flags bit 0) set. 0 restores the default of
+ * 2^22 stack-loop and DDA iterations, which no well-formed tree reaches;
+ * lower values force the guard, for testing failure reporting. */
+int svo_tree_set_step_limit(svo_tree* tree, uint32_t steps);
 uint64_t svo_tree_device_bytes(const svo_tree* dtree);
 
 /* Per-ray results (device arrays of n entries; any pointer may be NULL).
@@ -339,6 +345,8 @@ typedef struct svo_serve_config {
   double zipf_s;            /* Zipf exponent of the tree and resolution choice */
   uint32_t n_resolutions;
   const uint32_t* resolutions; /* n_resolutions (width, height) pairs, most popular first */
+  double deadline_ms;       /* requests not started this long after arrival are dropped
+                               (timed out; a dropped edit is not applied); 0 = none */
 } svo_serve_config;
 
 /* One request of a serving plan. Render viewports look at the tree's centre
@@ -368,6 +376,7 @@ typedef struct svo_serve_stats {
   /* phases of edit requests: arrival to start on the worker (waiting for the
    * tree to go idle), host octree edit, layout update and device upload */
   double edit_wait_p99_ms, edit_apply_p99_ms, edit_upload_p99_ms;
+  uint64_t timed_out;       /* requests dropped by the deadline (latency -2) */
 } svo_serve_stats;
 
 /* The request sequence of a serving session (deterministic in cfg and the
@@ -381,7 +390,8 @@ int svo_serve_apply_edit(svo_octree* tree, const svo_serve_request* req);
 /* Runs one serving session in real time over n_trees resident trees (each a
  * host octree and its device tree; edits change both). latencies_ms and
  * digests, when not NULL, receive one value per request of the plan
- * (capacity cap): latency (negative when rejected) and, for renders, a
+ * (capacity cap): latency (-1 when rejected, -2 when timed out) and, for
+ * renders, a
  * 64-bit digest of the result: the wrapping sum over pixels i = y*w+x of
  * m(...m(m(m(i + 0x9E3779B97F4A7C15) ^ hit) ^ albedo) ... ^ distance_bits)
  * over hit, albedo, voxel x/y/z and the f32 bits of distance, where m is

@@ -74,7 +74,8 @@ class ServeConfig(C.Structure):
                 ("max_batch_rays", C.c_uint64), ("max_batch_jobs", C.c_uint32),
                 ("max_queue", C.c_uint32), ("edit_fraction", C.c_double),
                 ("max_edit_voxels", C.c_uint32), ("zipf_s", C.c_double),
-                ("n_resolutions", C.c_uint32), ("resolutions", C.POINTER(C.c_uint32))]
+                ("n_resolutions", C.c_uint32), ("resolutions", C.POINTER(C.c_uint32)),
+                ("deadline_ms", C.c_double)]
 
 
 class ServeRequest(C.Structure):
@@ -93,7 +94,8 @@ class ServeStats(C.Structure):
                                              "mean_jobs_per_launch")]
                 + [("edit_records", C.c_uint64), ("edit_bricks", C.c_uint64)]
                 + [(k, C.c_double) for k in ("edit_wait_p99_ms", "edit_apply_p99_ms",
-                                             "edit_upload_p99_ms")])
+                                             "edit_upload_p99_ms")]
+                + [("timed_out", C.c_uint64)])
 
 
 _P = C.c_void_p
@@ -134,6 +136,7 @@ _SIGS = {
     "svo_tree_update_stats": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "svo_tree_destroy": (None, [_P]),
     "svo_tree_device_bytes": (C.c_uint64, [_P]),
+    "svo_tree_set_step_limit": (C.c_int, [_P, _U32]),
     "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),
     "svo_render": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.POINTER(Hits), _P]),
     "svo_render_rgba8": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, _U32, _P, _P]),

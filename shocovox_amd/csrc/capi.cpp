@@ -428,6 +428,13 @@ void svo_tree_destroy(svo_tree* dtree) {
   delete dtree;
 }
 
+int svo_tree_set_step_limit(svo_tree* dtree, uint32_t steps) {
+  if (!dtree || !dtree->dev) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  dtree->dev->step_limit = steps;
+  dtree->dev->view.max_steps = steps ? steps : svo::MAX_TRAVERSAL_STEPS;
+  return SVO_OK;
+}
+
 uint64_t svo_tree_device_bytes(const svo_tree* dtree) {
   return dtree && dtree->dev ? dtree->dev->bytes : 0;
 }

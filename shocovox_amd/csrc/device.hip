@@ -6,6 +6,7 @@
 //   the tree and share node and brick cache lines.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -194,6 +195,8 @@ const RenderVariant kRenderVariants[] = {
     {"w1-glut-8x8-n73", render_kernel<1, false, 8, 8, 73>, 8, 8},
     {"w1-glut-8x8-n585", render_kernel<1, false, 8, 8, 585>, 8, 8},
     {"w5-glut-8x8-n73-xcd", render_kernel<5, false, 8, 8, 73, true>, 8, 8},
+    {"w5-slut-8x8-n73", render_kernel<5, true, 8, 8, 73>, 8, 8},
+    {"w1-slut-8x8-n73", render_kernel<1, true, 8, 8, 73>, 8, 8},
     // lane refill measured 3.5x slower than one tile per wave on coherent
     // 1080p primary rays (lanes drift apart and lose the shared node lines);
     // kept as a variant for divergent workloads
@@ -430,6 +433,7 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   t->view.color_count = d.color_count;
   t->view.inv_dim = pow2_rcp(static_cast<float>(d.brick_dim));
   t->view.node_count = d.node_count;
+  t->view.max_steps = t->step_limit ? t->step_limit : MAX_TRAVERSAL_STEPS;
   return SVO_OK;
 }
 
@@ -489,66 +493,83 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
     }
   const uint32_t n3 = d.brick_dim * d.brick_dim * d.brick_dim;
   const uint32_t words = occ_words_per_brick(d.brick_dim);
-  // host staging: [records][record indices][brick words][slot indices]
-  std::vector<NodeRec> rec(records.size());
+  const uint32_t new_colors = d.color_count > colors_from ? d.color_count - colors_from : 0u;
+  // Staging layout (16-byte aligned sections), identical on host and device:
+  // [records][record indices][brick words][slot indices][new colours].
+  // The host side is pinned memory, so the whole patch is one asynchronous
+  // DMA on the caller's stream: a copy from pageable memory, or a hipFree,
+  // would wait for other streams' work and stall behind busy launches.
+  auto align16 = [](size_t v) { return (v + 15) & ~size_t(15); };
+  const size_t o_ridx = align16(records.size() * sizeof(NodeRec));
+  const size_t o_bw = align16(o_ridx + records.size() * 4);
+  const size_t o_slots = align16(o_bw + slots.size() * static_cast<size_t>(n3 + words) * 4);
+  const size_t o_colors = align16(o_slots + slots.size() * 4);
+  const size_t bytes = align16(o_colors + new_colors * 4u);
+  if (bytes > t->staging_bytes || bytes > t->host_staging_bytes) {
+    const size_t cap = std::max<size_t>(bytes + bytes / 2, size_t(8) << 20);
+    if (bytes > t->staging_bytes) {
+      // a smaller buffer is retired, not freed: freeing waits for the whole
+      // device; growth is geometric, so few buffers are ever retired
+      void* grown = nullptr;
+      e = hipMalloc(&grown, cap);
+      if (e != hipSuccess) return hip_fail(e, "hipMalloc(staging)", err);
+      if (t->staging) t->retired.push_back(t->staging);
+      t->staging = grown;
+      t->staging_bytes = cap;
+    }
+    if (bytes > t->host_staging_bytes) {
+      if (t->host_staging) {
+        // the previous patch synchronized its stream, so nothing reads it
+        (void)hipHostFree(t->host_staging);
+      }
+      t->host_staging = nullptr;
+      t->host_staging_bytes = 0;
+      e = hipHostMalloc(&t->host_staging, cap, hipHostMallocDefault);
+      if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)", err);
+      t->host_staging_bytes = cap;
+    }
+  }
+  char* h = static_cast<char*>(t->host_staging);
   for (size_t i = 0; i < records.size(); ++i)
-    std::memcpy(&rec[i], &d.nodes[records[i]], sizeof(NodeRec));
-  std::vector<uint32_t> bw(static_cast<size_t>(slots.size()) * (n3 + words));
+    std::memcpy(h + i * sizeof(NodeRec), &d.nodes[records[i]], sizeof(NodeRec));
+  if (!records.empty()) std::memcpy(h + o_ridx, records.data(), records.size() * 4);
   for (size_t i = 0; i < slots.size(); ++i) {
-    std::memcpy(&bw[i * (n3 + words)], d.brick_pix + static_cast<size_t>(slots[i]) * n3,
-                n3 * sizeof(uint32_t));
-    std::memcpy(&bw[i * (n3 + words) + n3], d.brick_occ + static_cast<size_t>(slots[i]) * words,
+    uint32_t* dst = reinterpret_cast<uint32_t*>(h + o_bw) + i * (n3 + words);
+    std::memcpy(dst, d.brick_pix + static_cast<size_t>(slots[i]) * n3, n3 * sizeof(uint32_t));
+    std::memcpy(dst + n3, d.brick_occ + static_cast<size_t>(slots[i]) * words,
                 words * sizeof(uint32_t));
   }
-  const size_t bytes = rec.size() * sizeof(NodeRec) + records.size() * 4 + bw.size() * 4 +
-                       slots.size() * 4 + 256;
-  if (bytes > t->staging_bytes) {
-    if (t->staging) {
-      (void)hipStreamSynchronize(s);
-      (void)hipFree(t->staging);
-      t->staging = nullptr;
-      t->staging_bytes = 0;
-    }
-    e = hipMalloc(&t->staging, bytes + bytes / 2);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(staging)", err);
-    t->staging_bytes = bytes + bytes / 2;
-  }
+  if (!slots.empty()) std::memcpy(h + o_slots, slots.data(), slots.size() * 4);
+  if (new_colors) std::memcpy(h + o_colors, d.colors + colors_from, new_colors * 4u);
   char* base = static_cast<char*>(t->staging);
-  NodeRec* d_rec = reinterpret_cast<NodeRec*>(base);
-  uint32_t* d_ridx = reinterpret_cast<uint32_t*>(d_rec + rec.size());
-  uint32_t* d_bw = d_ridx + records.size();
-  // keep the brick words 16-byte aligned
-  d_bw = reinterpret_cast<uint32_t*>((reinterpret_cast<uintptr_t>(d_bw) + 15) & ~uintptr_t(15));
-  uint32_t* d_slots = d_bw + bw.size();
+  if (bytes > 0) {
+    e = hipMemcpyAsync(base, h, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(patch)", err);
+  }
   if (!records.empty()) {
-    e = hipMemcpyAsync(d_rec, rec.data(), rec.size() * sizeof(NodeRec), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_ridx, records.data(), records.size() * 4, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(records)", err);
     const uint32_t n = static_cast<uint32_t>(records.size());
     hipLaunchKernelGGL(scatter_records_kernel, dim3((4 * n + 255) / 256), dim3(256), 0, s,
-                       t->nodes, d_rec, d_ridx, n);
+                       t->nodes, reinterpret_cast<const NodeRec*>(base),
+                       reinterpret_cast<const uint32_t*>(base + o_ridx), n);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "scatter_records_kernel", err);
   }
   if (!slots.empty()) {
-    e = hipMemcpyAsync(d_bw, bw.data(), bw.size() * 4, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(bricks)", err);
     const uint64_t total = static_cast<uint64_t>(slots.size()) * (n3 + words);
     hipLaunchKernelGGL(scatter_bricks_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)),
-                       dim3(256), 0, s, t->brick_pix, t->brick_occ, d_bw, d_slots,
+                       dim3(256), 0, s, t->brick_pix, t->brick_occ,
+                       reinterpret_cast<const uint32_t*>(base + o_bw),
+                       reinterpret_cast<const uint32_t*>(base + o_slots),
                        static_cast<uint32_t>(slots.size()), n3, words);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "scatter_bricks_kernel", err);
   }
-  if (d.color_count > colors_from) {
-    e = hipMemcpyAsync(t->colors + colors_from, d.colors + colors_from,
-                       (d.color_count - colors_from) * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+  if (new_colors) {
+    e = hipMemcpyAsync(t->colors + colors_from, base + o_colors, new_colors * 4u,
+                       hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(colors)", err);
   }
-  // the host staging vectors die with this call
+  // the pinned staging is reused by the next patch
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize", err);
   t->node_count = d.node_count;
@@ -563,6 +584,8 @@ void device_tree_destroy(DeviceTree* t) {
   if (!t) return;
   if (t->device >= 0) (void)hipSetDevice(t->device);
   (void)hipDeviceSynchronize();
+  if (t->host_staging) (void)hipHostFree(t->host_staging);
+  for (void* p : t->retired) (void)hipFree(p);
   if (t->nodes) (void)hipFree(t->nodes);
   if (t->brick_pix) (void)hipFree(t->brick_pix);
   if (t->brick_occ) (void)hipFree(t->brick_occ);

@@ -27,6 +27,10 @@ struct DeviceTree {
   uint64_t* ray_lut = nullptr;
   void* staging = nullptr;  // device staging of incremental patches
   uint64_t staging_bytes = 0;
+  void* host_staging = nullptr;  // pinned host staging of incremental patches
+  uint64_t host_staging_bytes = 0;
+  std::vector<void*> retired;    // outgrown device staging buffers, freed with the tree
+  uint32_t step_limit = 0;       // lowered iteration bound per ray (0: MAX_TRAVERSAL_STEPS)
   uint64_t bytes = 0;
   TreeView view{};
 };

@@ -7,7 +7,8 @@
 // runs out, instead of idling until the slowest ray of a fixed group ends.
 // The hand-out position is wave-uniform (it advances by the ballot count of
 // idle lanes), so the loop needs no atomics; each pass either hands out ids
-// or steps a ray that ends within MAX_TRAVERSAL_STEPS, so every wave leaves.
+// or steps a ray that ends within the tree's iteration bound, so every wave
+// leaves.
 #pragma once
 
 #include <hip/hip_runtime.h>

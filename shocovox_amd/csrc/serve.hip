@@ -93,6 +93,8 @@ constexpr double kDrainSeconds = 120.0;
 // of 8x8 pixels = 4096 rays), so that many waiting requests do not cut a
 // launch into slivers
 constexpr uint64_t kMinShareTiles = 64;
+// pause between polls of running launches and edits when nothing changed
+constexpr double kPollSeconds = 20e-6;
 
 int check_config(const svo_serve_config* cfg, uint32_t n_trees) {
   if (!cfg) return serve_fail(SVO_INVALID_ARGUMENT, "config is NULL");
@@ -114,6 +116,8 @@ int check_config(const svo_serve_config* cfg, uint32_t n_trees) {
   if (cfg->edit_fraction > 0.0 && cfg->max_edit_voxels == 0)
     return serve_fail(SVO_INVALID_ARGUMENT, "max_edit_voxels is 0");
   if (cfg->mode > 1) return serve_fail(SVO_INVALID_ARGUMENT, "mode must be 0 or 1");
+  if (!(cfg->deadline_ms >= 0.0) || !std::isfinite(cfg->deadline_ms))
+    return serve_fail(SVO_INVALID_ARGUMENT, "deadline_ms must be >= 0");
   if (cfg->max_inflight == 0 || cfg->max_inflight > 16)
     return serve_fail(SVO_INVALID_ARGUMENT, "max_inflight outside 1..16");
   if (cfg->mode == 0 && (cfg->max_batch_rays < 64 || cfg->max_batch_rays > (1ull << 30) ||
@@ -527,7 +531,8 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   std::vector<uint32_t> inflight(s.n_trees, 0);  // chunks launched, not finished
   std::vector<uint8_t> editing(s.n_trees, 0);
   std::vector<uint8_t> blocked(s.n_trees, 0), seen_render(s.n_trees, 0);
-  std::vector<uint8_t> started_edit(n, 0), dispatched(n, 0);
+  std::vector<uint8_t> started_edit(n, 0), dispatched(n, 0), timed_out(n, 0);
+  const double deadline = cfg.deadline_ms * 1e-3;
   std::vector<uint32_t> eligible;
   std::vector<uint64_t> take;
   std::deque<uint32_t> queue;
@@ -549,6 +554,8 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   const double drain_limit = (n ? s.plan[n - 1].arrival_s : 0.0) + kDrainSeconds;
   while (done < n && status == SVO_OK) {
     const double t = s.now();
+    const size_t done_before = done, arrivals_before = next_arrival;
+    const uint64_t launches_before = launches;
     if (t > drain_limit) {
       status = serve_fail(SVO_INVALID_ARGUMENT, "serving session did not drain");
       break;
@@ -616,10 +623,18 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
       // pass 1, in arrival order: start edits whose tree is idle, and list
       // the renders that may run now (not held behind an edit of their tree)
       eligible.clear();
+      const double tnow = s.now();
       for (auto it = queue.begin(); it != queue.end(); ++it) {
         const uint32_t r = *it;
         const svo_serve_request& q = s.plan[r];
         const uint32_t tr = q.tree;
+        // a request that has not started by its deadline is dropped
+        if (deadline > 0.0 && tnow - q.arrival_s > deadline &&
+            (q.kind == 1 || next_tile[r] == 0)) {
+          timed_out[r] = 1;
+          ++done;
+          continue;
+        }
         if (blocked[tr] || editing[tr]) {
           blocked[tr] = 1;
           continue;
@@ -682,7 +697,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
       }
       // drop started edits and fully dispatched renders from the queue
       for (auto it = queue.begin(); it != queue.end();) {
-        if (started_edit[*it] || dispatched[*it])
+        if (started_edit[*it] || dispatched[*it] || timed_out[*it])
           it = queue.erase(it);
         else
           ++it;
@@ -695,15 +710,22 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
       }
     }
     if (status != SVO_OK) break;
-    // nothing to do until the next arrival or completion
+    // Nothing happened in this pass: wait for the next arrival or completion
+    // without spinning on the runtime (a tight hipEventQuery loop contends
+    // with the edit worker's HIP calls and delays its uploads).
     bool busy = false;
     for (const Slot& sl : s.slots) busy = busy || sl.busy;
-    if (!busy && !changed && next_arrival < n) {
+    if (deadline > 0.0 && !queue.empty()) changed = true;  // deadlines pass with time
+    const bool progress =
+        done != done_before || next_arrival != arrivals_before || launches != launches_before;
+    if (!progress) {
       bool edits_running = false;
       for (uint8_t v : editing) edits_running = edits_running || v;
-      const double wait = s.plan[next_arrival].arrival_s - s.now();
-      if (!edits_running && wait > 2e-4)
+      const double wait = next_arrival < n ? s.plan[next_arrival].arrival_s - s.now() : 1.0;
+      if (!busy && !edits_running && !changed && wait > 2e-4)
         std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+      else if (wait > kPollSeconds)
+        std::this_thread::sleep_for(std::chrono::duration<double>(kPollSeconds));
     }
   }
 
@@ -728,10 +750,16 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   std::vector<double> all, ren, edi;
   double first = n ? s.plan[0].arrival_s : 0.0, last = first;
   uint64_t renders = 0, edits = 0, nrej = 0;
+  uint64_t ntimed = 0;
   for (size_t i = 0; i < n; ++i) {
     if (rejected[i]) {
       ++nrej;
       if (latencies && i < cap) latencies[i] = -1.0f;
+      continue;
+    }
+    if (timed_out[i]) {
+      ++ntimed;
+      if (latencies && i < cap) latencies[i] = -2.0f;
       continue;
     }
     const double lat = std::max(0.0, finish[i] - s.plan[i].arrival_s) * 1e3;
@@ -748,6 +776,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
     o.renders = renders;
     o.edits = edits;
     o.rejected = nrej;
+    o.timed_out = ntimed;
     o.rays = rays_done;
     o.launches = launches;
     o.elapsed_s = last - first;
@@ -771,7 +800,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
     o.edit_bricks = s.worker.bricks;
     std::vector<double> wait, apply, upload;
     for (size_t i = 0; i < n; ++i) {
-      if (s.plan[i].kind != 1 || rejected[i]) continue;
+      if (s.plan[i].kind != 1 || rejected[i] || timed_out[i]) continue;
       const EditTimes& et = s.worker.times[i];
       wait.push_back((et.start - s.plan[i].arrival_s) * 1e3);
       apply.push_back((et.applied - et.start) * 1e3);

@@ -39,6 +39,7 @@ struct TreeView {
   uint32_t color_count;
   float inv_dim;  // pow2_rcp(dim): exact since the brick dimension is a power of two
   uint32_t node_count;
+  uint32_t max_steps;  // iteration bound of one ray (MAX_TRAVERSAL_STEPS unless lowered)
 };
 
 struct CastResult {
@@ -177,7 +178,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
     count = count < 4u ? count + 1u : 4u;
     stk[head] = 0u;
     while (count != 0u) {
-      if (++steps > MAX_TRAVERSAL_STEPS) {
+      if (++steps > t.max_steps) {
         r.flags |= FLAG_ITERATION_GUARD;
         return r;
       }
@@ -227,8 +228,12 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
         continue;
       }
       Cube tb = child_bounds_for(bounds, target);
-      uint32_t tck = kind == KIND_INTERNAL ? n.child[target] : INVALID_INDEX;
-      if (tck != INVALID_INDEX && (occ & octant_bitmap_mask(target)) != 0ull) {
+      // the child word is read only when the occupancy test passes (pure
+      // reads: the same decision as testing the child first)
+      uint32_t tck = INVALID_INDEX;
+      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull)
+        tck = n.child[target];
+      if (tck != INVALID_INDEX) {
         // PUSH (:484-492)
         bounds = tb;
         target = hash_region(p - tb.min, tb.size / 2.0f);
@@ -238,23 +243,22 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
       } else {
         // ADVANCE (:497-544)
         for (;;) {
-          if (++steps > MAX_TRAVERSAL_STEPS) {
+          if (++steps > t.max_steps) {
             r.flags |= FLAG_ITERATION_GUARD;
             return r;
           }
           const V3f st = dda_step(d, sig, &p, tb, scale);
           target = step_octant(target, st);
-          if (target != OOB_OCTANT) {
-            tb = child_bounds_for(bounds, target);
-            tck = kind == KIND_INTERNAL ? n.child[target] : INVALID_INDEX;
-            const V3f sd = st * 4.0f;
-            bp = bp + div3_by(sd, bounds.size, rcp_size);
-            flat = bitmap_index(bp, &r.flags);
-          }
-          if (target == OOB_OCTANT ||
-              (tck != INVALID_INDEX && (occ & octant_bitmap_mask(target)) != 0ull &&
-               (lut[flat * 8u + dir_idx] & occ) != 0ull) ||
-              (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY))
+          if (target == OOB_OCTANT) break;
+          tb = child_bounds_for(bounds, target);
+          const V3f sd = st * 4.0f;
+          bp = bp + div3_by(sd, bounds.size, rcp_size);
+          flat = bitmap_index(bp, &r.flags);
+          // stop at a reachable occupied child or a non-empty leaf brick; the
+          // tests run cheapest first and the child word is read last
+          if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
+          if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
+              (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
             break;
         }
       }
@@ -341,7 +345,7 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
                                                           RayState& s, CastResult& r) {
   const float lo = 0.00001f;
   const float hi = 4.0f - 0.00001f;
-  if (++s.steps > MAX_TRAVERSAL_STEPS) {
+  if (++s.steps > t.max_steps) {
     r = miss_result(s.flags | FLAG_ITERATION_GUARD);
     return true;
   }
@@ -391,8 +395,10 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
     }
   } else {
     Cube tb = child_bounds_for(s.bounds, s.target);
-    uint32_t tck = kind == KIND_INTERNAL ? n.child[s.target] : INVALID_INDEX;
-    if (tck != INVALID_INDEX && (occ & octant_bitmap_mask(s.target)) != 0ull) {
+    uint32_t tck = INVALID_INDEX;
+    if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(s.target)) != 0ull)
+      tck = n.child[s.target];
+    if (tck != INVALID_INDEX) {
       // PUSH
       s.bounds = tb;
       s.target = hash_region(s.p - tb.min, tb.size / 2.0f);
@@ -402,23 +408,20 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
     } else {
       // ADVANCE
       for (;;) {
-        if (++s.steps > MAX_TRAVERSAL_STEPS) {
+        if (++s.steps > t.max_steps) {
           r = miss_result(s.flags | FLAG_ITERATION_GUARD);
           return true;
         }
         const V3f st = dda_step(s.d, sig, &s.p, tb, s.scale);
         s.target = step_octant(s.target, st);
-        if (s.target != OOB_OCTANT) {
-          tb = child_bounds_for(s.bounds, s.target);
-          tck = kind == KIND_INTERNAL ? n.child[s.target] : INVALID_INDEX;
-          const V3f sd = st * 4.0f;
-          bp = bp + div3_by(sd, s.bounds.size, rcp_size);
-          flat = bitmap_index(bp, &s.flags);
-        }
-        if (s.target == OOB_OCTANT ||
-            (tck != INVALID_INDEX && (occ & octant_bitmap_mask(s.target)) != 0ull &&
-             (lut[flat * 8u + dir_idx] & occ) != 0ull) ||
-            (kind == KIND_LEAF && brick_form(meta, s.target) != FORM_EMPTY))
+        if (s.target == OOB_OCTANT) break;
+        tb = child_bounds_for(s.bounds, s.target);
+        const V3f sd = st * 4.0f;
+        bp = bp + div3_by(sd, s.bounds.size, rcp_size);
+        flat = bitmap_index(bp, &s.flags);
+        if (kind == KIND_LEAF && brick_form(meta, s.target) != FORM_EMPTY) break;
+        if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(s.target)) != 0ull &&
+            (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[s.target] != INVALID_INDEX)
           break;
       }
     }

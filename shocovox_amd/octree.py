@@ -378,6 +378,11 @@ class DeviceTree:
             self._h = None
             _release("svo_tree_destroy", h)
 
+    def set_step_limit(self, steps: int) -> None:
+        """Iteration bound per ray (0: default); rays beyond it end as misses
+        with flag bit 0 set (svo_tree_set_step_limit)."""
+        _check(N.lib().svo_tree_set_step_limit(self._h, int(steps)))
+
     @property
     def device_bytes(self) -> int:
         return int(N.lib().svo_tree_device_bytes(self._h))

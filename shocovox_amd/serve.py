@@ -37,6 +37,7 @@ class ServeConfig:
     max_edit_voxels: int = 1000
     zipf_s: float = 1.1
     resolutions: tuple = field(default=SERVED_RESOLUTIONS)
+    deadline_ms: float = 0.0
 
     def to_c(self) -> tuple:
         res = (C.c_uint32 * (2 * len(self.resolutions)))(
@@ -45,7 +46,8 @@ class ServeConfig:
                           MODES[self.mode], int(self.max_inflight), int(self.max_batch_rays),
                           int(self.max_batch_jobs), int(self.max_queue),
                           float(self.edit_fraction), int(self.max_edit_voxels),
-                          float(self.zipf_s), len(self.resolutions), res)
+                          float(self.zipf_s), len(self.resolutions), res,
+                          float(self.deadline_ms))
         return c, res  # keep the resolution array alive with the struct
 
 
@@ -95,8 +97,9 @@ def apply_edit(tree: Octree, req: Request) -> None:
 
 
 def run(trees: list, dtrees: list, cfg: ServeConfig, digests: bool = False) -> dict:
-    """Runs a serving session; returns the stats, per-request latencies (ms,
-    negative when rejected) and, if asked, per-request result digests."""
+    """Runs a serving session; returns the stats, per-request latencies (ms;
+    -1 when rejected, -2 when timed out) and, if asked, per-request result
+    digests."""
     if len(trees) != len(dtrees) or not trees:
         raise ValueError("need one device tree per host tree")
     c, _res = cfg.to_c()

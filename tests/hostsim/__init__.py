@@ -18,7 +18,8 @@ HIT_DTYPE = np.dtype([("hit", "<u4"), ("pix", "<u4"), ("albedo", "<u4"), ("flags
 class _Tree(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("brick_pix", C.c_void_p), ("brick_occ", C.c_void_p),
                 ("colors", C.c_void_p), ("ray_lut", C.c_void_p), ("size", C.c_float),
-                ("dim", C.c_uint32), ("occ_words", C.c_uint32), ("color_count", C.c_uint32)]
+                ("dim", C.c_uint32), ("occ_words", C.c_uint32), ("color_count", C.c_uint32),
+                ("max_steps", C.c_uint32)]
 
 
 _lib = None
@@ -42,7 +43,8 @@ def ray_lut() -> np.ndarray:
     return out
 
 
-def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False) -> np.ndarray:
+def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False,
+         max_steps: int = 0) -> np.ndarray:
     """The product traversal on the host: cast_ray, or with resumable=True
     its ray_begin / ray_step form."""
     a = flat_arrays
@@ -50,7 +52,7 @@ def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False) -> np.nda
             np.ascontiguousarray(a["brick_occ"]), np.ascontiguousarray(a["colors"]), ray_lut()]
     dim = a["brick_dim"]
     t = _Tree(*(k.ctypes.data for k in keep), float(a["octree_size"]), dim,
-              (dim ** 3 + 31) // 32, len(a["colors"]))
+              (dim ** 3 + 31) // 32, len(a["colors"]), max_steps)
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
     out = np.zeros(len(rays), HIT_DTYPE)
     lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, int(resumable))

@@ -18,6 +18,7 @@ struct hs_tree {
   uint32_t dim;
   uint32_t occ_words;
   uint32_t color_count;
+  uint32_t max_steps;  // 0: the default bound
 };
 
 struct hs_hit {
@@ -32,7 +33,8 @@ struct hs_hit {
 // until the ray finishes), as the refill kernels run it.
 void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode) {
   svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut, t->size, t->dim,
-                  t->occ_words, t->color_count, svo::pow2_rcp(static_cast<float>(t->dim))};
+                  t->occ_words, t->color_count, svo::pow2_rcp(static_cast<float>(t->dim)), 0u,
+                  t->max_steps ? t->max_steps : svo::MAX_TRAVERSAL_STEPS};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
     const svo::V3f o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};

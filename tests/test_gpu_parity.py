@@ -361,3 +361,30 @@ def test_saved_tree_loaded_straight_to_device(gpu_device, tmp_path):
     got = gpu_to_numpy(d2.render(vp, cfg.width, cfg.height))
     torch.cuda.synchronize()
     assert_parity(_oracle(host).cast(rays), got, "loaded tree after edit")
+
+
+def test_forced_iteration_guard_on_device(gpu_device):
+    """svo_tree_set_step_limit: the device reports guard-ended rays exactly as
+    the host build of the same traversal does, and the default bound restores
+    oracle parity."""
+    import hostsim
+    import torch
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    d = DeviceTree(tree)
+    a = tree.flatten().arrays()
+    t = torch.from_numpy(rays).to(gpu_device)
+    for limit in (3, 0):
+        _say(f"step limit {limit}")
+        d.set_step_limit(limit)
+        got = gpu_to_numpy(d.cast(t))
+        torch.cuda.synchronize()
+        exp = hostsim.cast(a, rays, max_steps=limit)
+        assert_parity(exp, got, f"step limit {limit}")
+        if limit:
+            assert (got["flags"] & 1).any()
+        else:
+            assert (got["flags"] == 0).all()
+            assert_parity(_oracle(tree).cast(rays), got, "default bound")

@@ -127,3 +127,18 @@ def test_admission_by_free_device_memory(gpu_device):
     with pytest.raises(Exception) as e:
         serve.admit(t, reserve_bytes=total)
     assert "device bytes" in str(e.value)
+
+
+def test_served_deadline_drops_late_requests(gpu_device):
+    """Requests not started within the deadline are dropped (latency -2) and
+    counted; the rest complete."""
+    cfg = serve.ServeConfig(qps=20000, duration_s=0.05, seed=6, max_inflight=1,
+                            max_batch_rays=1 << 16, max_batch_jobs=8, deadline_ms=1.0,
+                            edit_fraction=0.2, max_edit_voxels=50, resolutions=((256, 256),))
+    trees = _trees()
+    _say("serve: deadline burst")
+    r = serve.run(trees, [DeviceTree(t) for t in trees], cfg)
+    lat = r["latencies_ms"]
+    assert r["timed_out"] > 0 and r["timed_out"] == int((lat == -2).sum())
+    assert r["renders"] + r["edits"] + r["timed_out"] + r["rejected"] == r["requests"]
+    assert (lat[lat >= 0] >= 0).all() and r["renders"] > 0

@@ -276,3 +276,26 @@ def test_resumable_traversal_equals_cast_ray():
         exp = oracle.OracleTree(t.export_model()).cast(rays)
         r = compare(exp, {k: res[k] for k in res.dtype.names})
         assert all(v == 0 for k, v in r.items() if k != "rays"), r
+
+
+def test_forced_iteration_guard_reports_and_ends_rays():
+    """A lowered iteration bound (the guard the reference lacks) ends long rays
+    as misses with flag bit 0 and leaves every other ray exactly as the
+    oracle has it."""
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width,
+                                cfg.height)[::7]
+    exp = oracle.OracleTree(tree.export_model()).cast(rays)
+    a = tree.flatten().arrays()
+    for limit in (1, 3, 6):
+        got = hostsim.cast(a, rays, max_steps=limit)
+        guarded = (got["flags"] & 1) == 1
+        assert guarded.any() and not guarded.all(), limit
+        assert (got["hit"][guarded] == 0).all()
+        ok = ~guarded
+        res = compare(exp[ok], {k: got[k][ok] for k in got.dtype.names})
+        assert all(v == 0 for k, v in res.items() if k != "rays"), (limit, res)
+    # the default bound is never reached on this workload
+    assert (hostsim.cast(a, rays)["flags"] == 0).all()

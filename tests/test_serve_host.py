@@ -86,7 +86,7 @@ def test_edit_replay_is_reproducible():
                                  dict(max_inflight=0), dict(max_inflight=17),
                                  dict(max_batch_rays=10), dict(max_batch_jobs=0),
                                  dict(edit_fraction=1.5), dict(resolutions=((0, 5),)),
-                                 dict(qps=1e9, duration_s=1.0)])
+                                 dict(qps=1e9, duration_s=1.0), dict(deadline_ms=-1.0)])
 def test_bad_configs_are_refused(bad):
     cfg = serve.ServeConfig(qps=100, duration_s=1.0)
     if bad.get("mode") == "x":
