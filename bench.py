@@ -282,7 +282,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="caves", choices=["caves", "plain"])
+    ap.add_argument("--workload", default="caves", choices=["caves", "plain", "spheres"],
+                    help="caves: the 512^3 1080p workload (default); spheres: the 2048^3 "
+                         "workload, one 3840x2160 frame plus 8M incoherent rays per step")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -335,9 +337,20 @@ def main() -> int:
     bufs = [alloc_hits(rays_per_frame, dev, fields) for _ in vps]
     stream = torch.cuda.current_stream()
 
+    incoherent = None
+    if cfg.name == "spheres":
+        # the 2048^3 workload adds 8M incoherent rays (origins in the volume,
+        # directions uniform on the sphere) to every step
+        inc = W.incoherent_rays(cfg.size, 8_000_000, seed=args.seed + 6)
+        incoherent = (torch.from_numpy(inc).to(dev), alloc_hits(len(inc), dev, fields))
+        rays_per_step += len(inc)
+        del inc
+
     def step():
         for vp, b in zip(vps, bufs):
             dtree.render_into(vp, cfg.width, cfg.height, b, stream)
+        if incoherent is not None:
+            dtree.cast_into(incoherent[0], incoherent[0].shape[0], incoherent[1], stream)
 
     names = render_variants()
     tuning = {}
@@ -351,6 +364,8 @@ def main() -> int:
             step()
             step()
             ms = collect_kernel_ms()
+            if incoherent is not None:
+                ms = ms[0::2]  # the render launches; the casts do not depend on the variant
             tuning[name] = round(sum(ms) / max(1, len(ms)), 4)
         set_timing(False)
         best = min(tuning, key=tuning.get)
@@ -371,7 +386,9 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     set_timing(False)
-    kernel_ms = collect_kernel_ms(cap=args.steps * len(vps) + 16)
+    kernel_ms = collect_kernel_ms(cap=args.steps * (len(vps) + 1) + 16)
+    if incoherent is not None:
+        kernel_ms = kernel_ms[0::2]  # render launches: the roofline prices the render kernel
     ms_per_step = elapsed / args.steps * 1e3
     value = rays_per_step * args.steps / elapsed / 1e6
     avg_launch_ms = sum(kernel_ms) / max(1, len(kernel_ms))
@@ -386,6 +403,7 @@ def main() -> int:
         b_ray = sum(EVENT_BYTES[k] * v for k, v in per_ray.items()) + OUT_BYTES_PER_RAY
         achieved = b_ray * rays_per_frame / (avg_launch_ms * 1e-3) / 1e9
         traffic = None
+        counters = {}
         # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
         # workload, scaled by the same counters' reading of a copy of known size
         tf = ROOT / "profiles" / f"render_kernel_{cfg.name}_{cfg.height}p_traffic.json"
@@ -393,13 +411,15 @@ def main() -> int:
             tj = json.loads(tf.read_text())
             if tj.get("units_per_launch") == rays_per_frame:
                 traffic = tj.get("bytes_per_launch")
+                counters = {k: tj[k] for k in ("l2_hit_rate", "valu_lane_utilization")
+                            if k in tj}
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "kernel": "render_kernel",
                     "bytes_per_ray": round(b_ray, 1),
                     "events_per_ray": {k: round(v, 3) for k, v in per_ray.items()},
                     "avg_launch_ms": round(avg_launch_ms, 4),
-                    "rays_per_launch": rays_per_frame}
+                    "rays_per_launch": rays_per_frame, **counters}
         log(f"cpu baseline {cpu}")
 
     line = {
