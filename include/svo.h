@@ -128,6 +128,20 @@ int svo_octree_from_bytes(const uint8_t* data, size_t len, svo_octree** out);
 int svo_octree_save(const svo_octree* tree, const char* path);
 int svo_octree_load(const char* path, svo_octree** out);
 
+/* MagicaVoxel import: Octree::load_vox_file(filename, brick_dimension)
+ * (convert/magicavoxel.rs:259-395). The tree's edge is 2^ceil(log2) of the
+ * scene's extent; every voxel that frame 0 of the scene graph places is
+ * inserted with its palette colour, converted from MagicaVoxel's
+ * right-handed z-up axes to the tree's left-handed y-up ones (x, z, y).
+ * Errors: SVO_INVALID_STRUCTURE for a malformed file, one without an RGBA
+ * palette chunk or with no model in frame 0; SVO_INVALID_SIZE /
+ * SVO_INVALID_BRICK_DIMENSION when no tree of that edge and brick
+ * dimension exists; SVO_INVALID_POSITION when a voxel lands outside the
+ * tree (each of these is a panic in the reference). */
+int svo_octree_from_vox_bytes(const uint8_t* data, size_t len, uint32_t brick_dim,
+                              svo_octree** out);
+int svo_octree_load_vox(const char* path, uint32_t brick_dim, svo_octree** out);
+
 /* ---------------------------------------------------------------------- */
 /* Synthetic scenes (the benchmark workloads), built with the host octree.  */
 /* ---------------------------------------------------------------------- */

@@ -124,6 +124,8 @@ _SIGS = {
     "svo_octree_from_bytes": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "svo_octree_save": (C.c_int, [_P, C.c_char_p]),
     "svo_octree_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "svo_octree_from_vox_bytes": (C.c_int, [_P, C.c_size_t, C.c_uint32, C.POINTER(_P)]),
+    "svo_octree_load_vox": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     "svo_tree_load": (C.c_int, [C.c_char_p, C.POINTER(_P), C.POINTER(_P)]),
     "svo_octree_flatten": (C.c_int, [_P, C.POINTER(_P)]),
     "svo_flat_desc": (C.c_int, [_P, C.POINTER(TreeDesc)]),

@@ -13,6 +13,7 @@
 #include "flatten.hpp"
 #include "handles.hpp"
 #include "octree.hpp"
+#include "vox.hpp"
 
 namespace svo {
 Status build_scene(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t seed, Octree* out);
@@ -347,6 +348,35 @@ int svo_octree_load(const char* path, svo_octree** out) {
   if (!f) return fail(SVO_INVALID_ARGUMENT, std::string("cannot read ") + path);
   const std::string b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
   return svo_octree_from_bytes(reinterpret_cast<const uint8_t*>(b.data()), b.size(), out);
+}
+
+int svo_octree_from_vox_bytes(const uint8_t* data, size_t len, uint32_t brick_dim,
+                              svo_octree** out) {
+  if (!out || (len && !data)) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  svo_octree* t = new (std::nothrow) svo_octree();
+  if (!t) return fail(SVO_INVALID_ARGUMENT, "out of host memory");
+  std::string err;
+  const svo::VoxStatus st = svo::octree_from_vox(data, len, brick_dim, &t->tree, &err);
+  if (st != svo::VoxStatus::Ok) {
+    delete t;
+    const int code = st == svo::VoxStatus::TreeSize   ? SVO_INVALID_SIZE
+                     : st == svo::VoxStatus::BrickDim ? SVO_INVALID_BRICK_DIMENSION
+                     : st == svo::VoxStatus::Position ? SVO_INVALID_POSITION
+                                                      : SVO_INVALID_STRUCTURE;
+    return fail(code, "vox import: " + err);
+  }
+  *out = t;
+  return SVO_OK;
+}
+
+int svo_octree_load_vox(const char* path, uint32_t brick_dim, svo_octree** out) {
+  if (!path || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(SVO_INVALID_ARGUMENT, std::string("cannot read ") + path);
+  const std::string b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  return svo_octree_from_vox_bytes(reinterpret_cast<const uint8_t*>(b.data()), b.size(),
+                                   brick_dim, out);
 }
 
 int svo_tree_load(const char* path, svo_octree** host_out, svo_tree** out) {

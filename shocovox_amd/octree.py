@@ -126,6 +126,24 @@ class Octree:
         _check(N.lib().svo_octree_load(str(path).encode(), C.byref(h)))
         return cls(0, 0, _handle=h.value)
 
+    @classmethod
+    def load_vox_file(cls, filename: str, brick_dimension: int) -> "Octree":
+        """Octree::load_vox_file (convert/magicavoxel.rs:259-278): a MagicaVoxel
+        scene, frame 0, in a tree of edge 2^ceil(log2(extent))."""
+        h = C.c_void_p()
+        _check(N.lib().svo_octree_load_vox(str(filename).encode(), int(brick_dimension),
+                                           C.byref(h)))
+        return cls(0, 0, _handle=h.value)
+
+    @classmethod
+    def from_vox_bytes(cls, data: bytes, brick_dimension: int) -> "Octree":
+        """load_vox_file on a file image already in memory."""
+        h = C.c_void_p()
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data) if data else None
+        _check(N.lib().svo_octree_from_vox_bytes(buf, len(data), int(brick_dimension),
+                                                 C.byref(h)))
+        return cls(0, 0, _handle=h.value)
+
     def to_bytes(self) -> bytes:
         """Octree::to_bytes (octree/mod.rs:136-140)."""
         n = C.c_size_t()

@@ -388,3 +388,33 @@ def test_forced_iteration_guard_on_device(gpu_device):
         else:
             assert (got["flags"] == 0).all()
             assert_parity(_oracle(tree).cast(rays), got, "default bound")
+
+
+@pytest.mark.parametrize("name,dim", [("navigate", 8), ("navigate_x", 2), ("navigate_z", 32)])
+def test_vox_scene_renders_like_the_oracle(gpu_device, name, dim):
+    """A MagicaVoxel asset imported with Octree::load_vox_file, flattened and
+    uploaded: a look-at render and 100k incoherent rays equal the oracle."""
+    import torch
+    from pathlib import Path
+    path = Path(__file__).resolve().parent / "golden" / "vox" / f"{name}.vox"
+    from oracle import vox as OV
+    tree = Octree.load_vox_file(path, dim)
+    s = float(tree.get_size())
+    pts = np.array([p[:3] for p in OV.load(path.read_bytes(), dim)[1]], np.float64)
+    lo, hi = pts.min(0), pts.max(0) + 1.0
+    c, ext = (lo + hi) / 2.0, float((hi - lo).max())
+    vp = W.look_at_viewport(tuple(c + np.array([0.9, 0.6, 1.1]) * ext), tuple(c), 640, 360,
+                            vfov_deg=28.0)
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 640, 360)
+    _say(f"vox {name} dim {dim} render")
+    d = DeviceTree(tree)
+    got = gpu_to_numpy(d.render(vp, 640, 360))
+    torch.cuda.synchronize()
+    exp = _oracle(tree).cast(rays)
+    assert exp["hit"].sum() > 1000
+    assert_parity(exp, got, f"vox {name} render")
+    inc = W.incoherent_rays(int(s), 100_000, seed=17)
+    _say(f"vox {name} dim {dim} incoherent")
+    got = gpu_to_numpy(d.cast(torch.from_numpy(inc).to(gpu_device)))
+    torch.cuda.synchronize()
+    assert_parity(_oracle(tree).cast(inc), got, f"vox {name} incoherent")
