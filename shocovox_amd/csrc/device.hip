@@ -338,6 +338,11 @@ namespace {
 // Capacity an upload allocates for `need` elements: a quarter of headroom so
 // that edits which add nodes or bricks are patched in place.
 uint64_t grown_capacity(uint64_t need) { return need + need / 4 + 1; }
+// Palettes grow with almost every edit that brings a new colour, and an entry
+// costs 4 bytes, so they get a fixed allowance on top.
+uint64_t grown_color_capacity(uint64_t need) { return grown_capacity(need) + 4096; }
+// A tree that outgrew its arrays is being edited: it grows by half again.
+uint64_t regrown_capacity(uint64_t need) { return need + need / 2 + 64; }
 }  // namespace
 
 uint64_t device_tree_bytes_for(const svo_tree_desc& d) {
@@ -345,7 +350,7 @@ uint64_t device_tree_bytes_for(const svo_tree_desc& d) {
   const uint64_t words = occ_words_per_brick(d.brick_dim);
   const uint64_t bricks = grown_capacity(d.brick_count ? d.brick_count : 1);
   return grown_capacity(d.node_count) * sizeof(NodeRec) + bricks * (n3 + words) * sizeof(uint32_t) +
-         grown_capacity(d.color_count ? d.color_count : 1) * sizeof(uint32_t) +
+         grown_color_capacity(d.color_count ? d.color_count : 1) * sizeof(uint32_t) +
          64 * 8 * sizeof(uint64_t);
 }
 
@@ -358,7 +363,8 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   const uint64_t words = occ_words_per_brick(d.brick_dim);
   auto grow = [&](void** p, uint64_t* cap, uint64_t need, uint64_t elem) -> int {
     if (need <= *cap && *p) return SVO_OK;
-    uint64_t nc = grown_capacity(need);
+    uint64_t nc = p == reinterpret_cast<void**>(&t->colors) ? grown_color_capacity(need)
+                                                             : grown_capacity(need);
     void* np = nullptr;
     hipError_t ge = hipMalloc(&np, nc * elem);
     if (ge != hipSuccess) return hip_fail(ge, "hipMalloc", err);
@@ -472,12 +478,107 @@ int device_tree_create(const svo_tree_desc& d, DeviceTree** out, std::string* er
   return SVO_OK;
 }
 
+namespace {
+
+// Moves one device array into a larger allocation: the live prefix is copied
+// device to device on `s`; the old buffer is retired (kernels already queued
+// may still read it). Retired memory beyond the tree's own size is returned
+// to the device first, which waits for the device once.
+int regrow(DeviceTree* t, void** p, uint64_t* cap, uint64_t new_cap, uint64_t live_bytes,
+           uint64_t elem, hipStream_t s, std::string* err) {
+  if (t->retired_bytes > std::max<uint64_t>(t->bytes, uint64_t(64) << 20)) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize", err);
+    for (void* q : t->retired) (void)hipFree(q);
+    t->retired.clear();
+    t->retired_bytes = 0;
+  }
+  void* np = nullptr;
+  hipError_t e = hipMalloc(&np, new_cap * elem);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(grow)", err);
+  if (*p && live_bytes) {
+    e = hipMemcpyAsync(np, *p, live_bytes, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+      (void)hipFree(np);
+      return hip_fail(e, "hipMemcpyAsync(grow)", err);
+    }
+  }
+  if (*p) {
+    t->retired.push_back(*p);
+    t->retired_bytes += *cap * elem;
+  }
+  *p = np;
+  *cap = new_cap;
+  return SVO_OK;
+}
+
+// Grows the arrays of a resident tree to hold `d`, keeping their contents.
+int grow_resident(DeviceTree* t, const svo_tree_desc& d, hipStream_t s, std::string* err) {
+  const uint64_t n3 = static_cast<uint64_t>(d.brick_dim) * d.brick_dim * d.brick_dim;
+  const uint64_t words = occ_words_per_brick(d.brick_dim);
+  int st = SVO_OK;
+  if (d.node_count > t->node_capacity)
+    st = regrow(t, reinterpret_cast<void**>(&t->nodes), &t->node_capacity,
+                regrown_capacity(d.node_count), t->node_count * sizeof(NodeRec), sizeof(NodeRec), s,
+                err);
+  if (st == SVO_OK && d.brick_count > t->brick_capacity) {
+    const uint64_t nc = regrown_capacity(d.brick_count);
+    uint64_t occ_cap = t->brick_capacity;
+    st = regrow(t, reinterpret_cast<void**>(&t->brick_pix), &t->brick_capacity, nc,
+                t->brick_count * n3 * 4, n3 * 4, s, err);
+    if (st == SVO_OK)
+      st = regrow(t, reinterpret_cast<void**>(&t->brick_occ), &occ_cap, nc,
+                  t->brick_count * words * 4, words * 4, s, err);
+  }
+  if (st == SVO_OK && d.color_count > t->color_capacity)
+    st = regrow(t, reinterpret_cast<void**>(&t->colors), &t->color_capacity,
+                grown_color_capacity(d.color_count), t->color_count * 4ull, 4, s, err);
+  if (st != SVO_OK) return st;
+  t->bytes = t->node_capacity * sizeof(NodeRec) + t->brick_capacity * n3 * sizeof(uint32_t) +
+             t->brick_capacity * words * sizeof(uint32_t) + t->color_capacity * sizeof(uint32_t) +
+             64 * 8 * sizeof(uint64_t);
+  t->view.nodes = t->nodes;
+  t->view.brick_pix = t->brick_pix;
+  t->view.brick_occ = t->brick_occ;
+  t->view.colors = t->colors;
+  return SVO_OK;
+}
+
+}  // namespace
+
+int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err) {
+  if (bytes <= t->staging_bytes && bytes <= t->host_staging_bytes) return SVO_OK;
+  hipError_t e = hipSetDevice(t->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
+  const size_t cap = std::max<size_t>(bytes + bytes / 2, size_t(8) << 20);
+  if (bytes > t->staging_bytes) {
+    // a smaller buffer is retired, not freed: freeing waits for the whole
+    // device; growth is geometric, so few buffers are ever retired
+    void* grown = nullptr;
+    e = hipMalloc(&grown, cap);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(staging)", err);
+    if (t->staging) t->retired.push_back(t->staging);
+    t->staging = grown;
+    t->staging_bytes = cap;
+  }
+  if (bytes > t->host_staging_bytes) {
+    if (t->host_staging) {
+      // the previous patch synchronized its stream, so nothing reads it
+      (void)hipHostFree(t->host_staging);
+    }
+    t->host_staging = nullptr;
+    t->host_staging_bytes = 0;
+    e = hipHostMalloc(&t->host_staging, cap, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)", err);
+    t->host_staging_bytes = cap;
+  }
+  return SVO_OK;
+}
+
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<uint32_t>& records,
                       const std::vector<uint32_t>& slots, uint32_t colors_from, void* stream,
                       std::string* err) {
-  if (d.node_count > t->node_capacity || (d.brick_count > t->brick_capacity) ||
-      d.color_count > t->color_capacity || d.brick_dim != t->view.dim)
-    return 1;  // caller re-uploads everything
+  if (d.brick_dim != t->view.dim) return 1;  // caller re-uploads everything
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(t->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
@@ -491,6 +592,11 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
       *err = "patched brick slot out of range";
       return SVO_INVALID_ARGUMENT;
     }
+  if (d.node_count > t->node_capacity || d.brick_count > t->brick_capacity ||
+      d.color_count > t->color_capacity) {
+    const int st = grow_resident(t, d, s, err);
+    if (st != SVO_OK) return st;
+  }
   const uint32_t n3 = d.brick_dim * d.brick_dim * d.brick_dim;
   const uint32_t words = occ_words_per_brick(d.brick_dim);
   const uint32_t new_colors = d.color_count > colors_from ? d.color_count - colors_from : 0u;
@@ -505,30 +611,8 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   const size_t o_slots = align16(o_bw + slots.size() * static_cast<size_t>(n3 + words) * 4);
   const size_t o_colors = align16(o_slots + slots.size() * 4);
   const size_t bytes = align16(o_colors + new_colors * 4u);
-  if (bytes > t->staging_bytes || bytes > t->host_staging_bytes) {
-    const size_t cap = std::max<size_t>(bytes + bytes / 2, size_t(8) << 20);
-    if (bytes > t->staging_bytes) {
-      // a smaller buffer is retired, not freed: freeing waits for the whole
-      // device; growth is geometric, so few buffers are ever retired
-      void* grown = nullptr;
-      e = hipMalloc(&grown, cap);
-      if (e != hipSuccess) return hip_fail(e, "hipMalloc(staging)", err);
-      if (t->staging) t->retired.push_back(t->staging);
-      t->staging = grown;
-      t->staging_bytes = cap;
-    }
-    if (bytes > t->host_staging_bytes) {
-      if (t->host_staging) {
-        // the previous patch synchronized its stream, so nothing reads it
-        (void)hipHostFree(t->host_staging);
-      }
-      t->host_staging = nullptr;
-      t->host_staging_bytes = 0;
-      e = hipHostMalloc(&t->host_staging, cap, hipHostMallocDefault);
-      if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)", err);
-      t->host_staging_bytes = cap;
-    }
-  }
+  const int sst = device_tree_reserve_staging(t, bytes, err);
+  if (sst != SVO_OK) return sst;
   char* h = static_cast<char*>(t->host_staging);
   for (size_t i = 0; i < records.size(); ++i)
     std::memcpy(h + i * sizeof(NodeRec), &d.nodes[records[i]], sizeof(NodeRec));

@@ -29,7 +29,8 @@ struct DeviceTree {
   uint64_t staging_bytes = 0;
   void* host_staging = nullptr;  // pinned host staging of incremental patches
   uint64_t host_staging_bytes = 0;
-  std::vector<void*> retired;    // outgrown device staging buffers, freed with the tree
+  std::vector<void*> retired;    // outgrown device buffers, freed with the tree
+  uint64_t retired_bytes = 0;
   uint32_t step_limit = 0;       // lowered iteration bound per ray (0: MAX_TRAVERSAL_STEPS)
   uint64_t bytes = 0;
   TreeView view{};
@@ -44,10 +45,14 @@ void device_tree_destroy(DeviceTree* t);
 // Device bytes a fresh device_tree_create of `desc` allocates (capacities
 // include the growth headroom of uploads).
 uint64_t device_tree_bytes_for(const svo_tree_desc& desc);
-// Writes the given records and brick slots of a layout into an existing tree
-// whose capacities already cover `desc` (the layout's full description);
-// returns SVO_OUT_OF_DEVICE_MEMORY-free status or asks for a full upload by
-// returning 1 when the capacities are too small.
+// Writes the given records and brick slots of a layout into an existing tree.
+// Arrays whose capacity `desc` (the layout's full description) outgrows are
+// reallocated larger and their live contents copied device to device on
+// `stream` first, so a patch never re-sends the whole tree from the host.
+// Returns 1 (asking for a full upload) only when the brick dimension differs.
+// Makes the patch staging (pinned host and device) hold at least `bytes`
+// (at least 8 MiB), so the first edits of a tree allocate nothing.
+int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err);
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& desc,
                       const std::vector<uint32_t>& records, const std::vector<uint32_t>& slots,
                       uint32_t colors_from, void* stream, std::string* err);

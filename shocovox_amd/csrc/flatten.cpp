@@ -42,7 +42,7 @@ void parallel_range(size_t n, F&& f) {
   for (auto& th : pool) th.join();
 }
 
-FlatTree flatten(const Octree& tree) {
+FlatTree flatten(const Octree& tree, bool headroom) {
   FlatTree f;
   f.octree_size = tree.size();
   f.brick_dim = tree.brick_dim();
@@ -79,6 +79,8 @@ FlatTree flatten(const Octree& tree) {
 
   // Records, with Parted bricks numbered in record order; the brick words
   // are then copied and their occupancy sidecars built in parallel.
+  auto with_room = [headroom](size_t n) { return headroom ? n + n / 4 + 64 : n; };
+  f.nodes.reserve(with_room(order.size()));
   f.nodes.resize(order.size());
   std::vector<const std::vector<uint32_t>*> parted;
   auto encode_brick = [&](const Brick& b, uint32_t octant, NodeRec& r) {
@@ -123,6 +125,8 @@ FlatTree flatten(const Octree& tree) {
         break;
     }
   }
+  f.brick_pix.reserve(with_room(parted.size()) * n3);
+  f.brick_occ.reserve(with_room(parted.size()) * words);
   f.brick_pix.resize(parted.size() * n3);
   f.brick_occ.assign(parted.size() * words, 0u);
   parallel_range(parted.size(), [&](size_t slot) {
@@ -221,7 +225,8 @@ constexpr uint32_t NO_RECORD = 0xFFFFFFFFu;
 }
 
 void LayoutMirror::build(const Octree& tree) {
-  flat_ = flatten(tree);
+  flat_ = flatten(tree, true);
+  key_to_rec_.reserve(tree.nodes().len() + tree.nodes().len() / 4 + 64);
   key_to_rec_.assign(tree.nodes().len(), NO_RECORD);
   // recover key -> record from the breadth-first order flatten() used
   std::vector<uint32_t> order{Octree::ROOT_NODE_KEY};
@@ -240,7 +245,10 @@ void LayoutMirror::build(const Octree& tree) {
   free_slots_.clear();
   dirty_recs_.clear();
   dirty_slots_.clear();
+  rec_dirty_flag_.reserve(flat_.nodes.capacity());
   rec_dirty_flag_.assign(flat_.nodes.size(), 0);
+  const size_t n3 = static_cast<size_t>(flat_.brick_dim) * flat_.brick_dim * flat_.brick_dim;
+  slot_dirty_flag_.reserve(n3 ? flat_.brick_pix.capacity() / n3 : 0);
   slot_dirty_flag_.assign(flat_.brick_count(), 0);
   full_ = true;
   colors_synced_ = 0;

@@ -25,7 +25,10 @@ struct FlatTree {
 
 // Breadth-first renumbering of every node reachable from the root through
 // valid child keys; bricks are copied into slots in visiting order.
-FlatTree flatten(const Octree& tree);
+// With `headroom`, the node and brick arrays reserve a quarter more room
+// (address space only until used), so a mirror that grows them edit by edit
+// does not copy the whole layout on its first new record or brick.
+FlatTree flatten(const Octree& tree, bool headroom = false);
 
 // Point query on the flattened layout; it must agree with Octree::get for
 // every position (used as the layout round-trip check). Returns the palette

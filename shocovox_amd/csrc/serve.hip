@@ -539,6 +539,19 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   size_t next_arrival = 0, done = 0;
   uint64_t launches = 0, chunks_launched = 0, rays_done = 0;
 
+  // every tree that an edit of the plan touches gets its patch staging now,
+  // so no edit allocates pinned or device memory inside the session
+  {
+    std::vector<uint8_t> edited(s.n_trees, 0);
+    for (const svo_serve_request& q : s.plan)
+      if (q.kind == 1) edited[q.tree] = 1;
+    for (uint32_t i = 0; i < s.n_trees; ++i) {
+      if (!edited[i]) continue;
+      std::string err;
+      if (device_tree_reserve_staging(s.dtrees[i]->dev, 1, &err) != SVO_OK)
+        return serve_fail(SVO_HIP_ERROR, "edit staging: " + err);
+    }
+  }
   hipError_t e = hipEventCreate(&s.ref);
   if (e != hipSuccess) return hip_err(e, "hipEventCreate");
   s.worker.times.assign(n, EditTimes{});

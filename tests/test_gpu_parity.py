@@ -225,6 +225,49 @@ def test_in_place_edits_on_device(gpu_device):
         assert_parity(exp, gpu_to_numpy(bufs), f"edit round {rnd}")
 
 
+def test_patch_that_outgrows_the_device_arrays(gpu_device):
+    """Edits that need more records, brick slots and palette entries than the
+    device arrays hold grow them on the device (old contents copied device to
+    device, old buffers kept until the tree is destroyed) instead of
+    re-sending the tree; renders equal the oracle after every round, and a
+    render queued on another stream before the growing update still sees the
+    tree as it was."""
+    import torch
+    from shocovox_amd import OctreeEntry
+    tree = Octree.new(64, 4)
+    tree.insert((3, 3, 3), OctreeEntry.visual(0x102030FF))
+    d = DeviceTree(tree)
+    s = float(tree.get_size())
+    vp = W.look_at_viewport((1.6 * s, 1.1 * s, 1.4 * s), (s / 2, s / 2, s / 2), 320, 200)
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 320, 200)
+    rng = np.random.default_rng(29)
+    side = torch.cuda.Stream()
+    for rnd in range(3):
+        before = d.device_bytes
+        exp_prev = _oracle(tree).cast(rays)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            queued = d.render(vp, 320, 200, stream=side)
+        n = 3000 * (rnd + 1)
+        xyz = rng.integers(0, 64, size=(n, 3)).astype(np.uint32)
+        ents = np.zeros((n, 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = (rng.integers(1, 1 << 24, n).astype(np.uint32) << 8) | 0xFF
+        tree.insert_many(xyz, ents)
+        if rnd > 0:
+            side.synchronize()  # without growth a patch rewrites the arrays in place
+        recs, slots = d.update(tree)
+        _say(f"growth round {rnd}: {recs} records, {slots} bricks, "
+             f"{before} -> {d.device_bytes} device bytes")
+        side.synchronize()
+        if rnd == 0:
+            assert d.device_bytes > before
+        assert_parity(exp_prev, gpu_to_numpy(queued), f"render queued before round {rnd}")
+        got = gpu_to_numpy(d.render(vp, 320, 200))
+        torch.cuda.synchronize()
+        assert_parity(_oracle(tree).cast(rays), got, f"growth round {rnd}")
+
+
 @pytest.mark.parametrize("variant", range(4))
 def test_batched_mixed_jobs_and_replays(gpu_device, variant):
     """Many trees, uneven viewports and ray arrays in one persistent launch;
