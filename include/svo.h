@@ -100,6 +100,8 @@ typedef struct svo_model_sizes {
   uint32_t parted_count;
   uint32_t color_count;
   uint32_t data_count;
+  uint32_t mip_parted_count; /* Parted MIP bricks */
+  uint32_t mips_enabled;     /* MIPMapStrategy::enabled */
 } svo_model_sizes;
 
 typedef struct svo_model_dump {
@@ -114,9 +116,44 @@ typedef struct svo_model_dump {
   uint32_t* voxels;          /* [parted_count * dim^3] */
   uint32_t* colors;          /* [color_count] */
   uint32_t* data;            /* [data_count] */
+  uint8_t* mip_kind;         /* [pool_len]: node_mips, as brick_kind */
+  uint32_t* mip_value;       /* [pool_len]: Solid value or Parted index in mip_voxels */
+  uint32_t* mip_voxels;      /* [mip_parted_count * dim^3] */
 } svo_model_dump;
 
 int svo_octree_export(const svo_octree* tree, svo_model_sizes* sizes, svo_model_dump* dump);
+
+/* MIP maps (src/octree/mipmap.rs). Methods are MIPResamplingMethods:
+ * 0 BoxFilter, 1 PointFilter, 2 PointFilterBD, 3 Posterize(thr),
+ * 4 PosterizeBD(thr); thresholds are clamped to [0, 1]. Where the reference
+ * picks among equally frequent colours through a HashMap (PointFilter,
+ * Posterize), whose order varies from run to run, the colour met first in
+ * sampling order is taken. */
+typedef enum svo_mip_method {
+  SVO_MIP_BOX_FILTER = 0,
+  SVO_MIP_POINT_FILTER = 1,
+  SVO_MIP_POINT_FILTER_BD = 2,
+  SVO_MIP_POSTERIZE = 3,
+  SVO_MIP_POSTERIZE_BD = 4
+} svo_mip_method;
+/* StrategyUpdater::switch_albedo_mip_maps (mipmap.rs:865-880): turning MIPs
+ * on recalculates every MIP of a non-empty tree */
+int svo_octree_switch_mip_maps(svo_octree* tree, int enabled);
+int svo_octree_mip_maps_enabled(const svo_octree* tree);
+/* set_method_at (mipmap.rs:760-765), get_method_at (:754-757) */
+int svo_octree_set_mip_method(svo_octree* tree, uint32_t level, uint32_t method, float thr);
+int svo_octree_get_mip_method(const svo_octree* tree, uint32_t level, uint32_t* method,
+                              float* thr);
+/* set_color_similarity_thr_at (:737-742), get_new_color_similarity_at (:729-734) */
+int svo_octree_set_mip_color_threshold(svo_octree* tree, uint32_t level, float thr);
+int svo_octree_get_mip_color_threshold(const svo_octree* tree, uint32_t level, float* thr);
+/* reset (:721-724): the default strategy (disabled) */
+int svo_octree_reset_mip_strategy(svo_octree* tree);
+/* recalculate_mips (:798-862) */
+int svo_octree_recalculate_mips(svo_octree* tree);
+/* sample_root_mip (:905-937): octant 8 samples the root's MIP, 0-7 a child's */
+int svo_octree_sample_root_mip(const svo_octree* tree, uint32_t octant, uint32_t x, uint32_t y,
+                               uint32_t z, svo_entry* out);
 
 /* Persisted trees in the reference's format: the bencode of Octree<u32>
  * that Octree::save / Octree::to_bytes write (octree/mod.rs:136-168,
@@ -162,13 +199,16 @@ int svo_scene_build(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t s
 
 typedef struct svo_tree svo_tree;
 
-/* One 64-byte node record of the device layout. */
+/* One 64-byte node record of the device layout. `mip` holds the form of the
+ * node's MIP brick (0 Empty, 1 Solid, 2 Parted) and `mip_word` its Solid value
+ * or Parted brick slot. */
 typedef struct svo_node_rec {
   uint64_t occ;
   uint32_t meta;
-  uint32_t reserved0;
+  uint32_t mip;
   uint32_t child[8];
-  uint32_t reserved1[4];
+  uint32_t mip_word;
+  uint32_t reserved[3];
 } svo_node_rec;
 
 /* Host description of a flattened tree: what a binding that owns its own
@@ -184,6 +224,7 @@ typedef struct svo_tree_desc {
   const uint32_t* brick_pix; /* brick_count * dim^3 palette index values */
   const uint32_t* brick_occ; /* brick_count * ceil(dim^3/32) occupancy words */
   const uint32_t* colors;    /* color_count packed albedo values */
+  uint32_t mips_enabled;     /* MIPMapStrategy::enabled: LOD queries consult MIPs */
 } svo_tree_desc;
 
 /* Flattened layout of a host octree, kept on the host; owned by the handle. */
@@ -264,6 +305,17 @@ typedef struct svo_viewport {
  * harness's ray convention (examples/cpu_render.rs:78-114). */
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream);
+
+/* Octree::get_by_ray_at_lod (src/raytracing/raytracing_on_cpu.rs:325-565)
+ * for n rays / for a viewport's primary rays: when the tree's MIP strategy
+ * is enabled, a node's MIP brick answers for its subtree once the node's
+ * level is below the distance the ray has travelled divided by
+ * viewing_distance (> 0). With MIPs disabled the results equal svo_cast /
+ * svo_render. */
+int svo_cast_at_lod(const svo_tree* dtree, const float* rays, uint32_t n,
+                    float viewing_distance, const svo_hits* out, void* stream);
+int svo_render_at_lod(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
+                      float viewing_distance, const svo_hits* out, void* stream);
 
 /* Shaded render to RGBA8, the drop-in for the image the reference draws:
  * out holds w*h u32 pixels (r in the low byte, alpha 255), row 0 at the top.

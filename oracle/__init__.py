@@ -27,7 +27,9 @@ class _Dump(C.Structure):
                [(n, C.c_void_p) for n in ("reserved", "content_kind", "internal_bits",
                                           "children_kind", "children", "children_bitmap",
                                           "brick_kind", "brick_value", "voxels", "colors",
-                                          "data")]
+                                          "data")] + \
+               [(n, C.c_uint32) for n in ("mip_parted_count", "mips_enabled")] + \
+               [(n, C.c_void_p) for n in ("mip_kind", "mip_value", "mip_voxels")]
 
 
 class Counters(C.Structure):
@@ -58,6 +60,8 @@ def lib() -> C.CDLL:
         L.ora_get.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
         L.ora_get_many.argtypes = [P, P, C.c_uint64, P]
         L.ora_cast.argtypes = [P, P, C.c_uint64, P, C.c_uint32, C.POINTER(Counters)]
+        L.ora_cast_lod.argtypes = [P, P, C.c_uint64, P, C.c_uint32, C.POINTER(Counters),
+                                   C.c_float]
         L.ora_viewport_rays.argtypes = [P, C.c_uint32, C.c_uint32, P]
         L.ora_luts.argtypes = [P, P, P, P]
         L.ora_intersect_ray.restype = C.c_int
@@ -94,7 +98,10 @@ class OracleTree:
                   *(self._keep[k].ctypes.data for k in (
                       "reserved", "content_kind", "internal_bits", "children_kind", "children",
                       "children_bitmap", "brick_kind", "brick_value", "voxels", "colors",
-                      "data")))
+                      "data")),
+                  model.get("mip_parted_count", 0), model.get("mips_enabled", 0),
+                  *((self._keep[k].ctypes.data if k in self._keep else None)
+                    for k in ("mip_kind", "mip_value", "mip_voxels")))
         self._h = lib().ora_tree_load(C.byref(d))
         self._keep = None  # the oracle copied what it needs
 
@@ -114,13 +121,19 @@ class OracleTree:
         lib().ora_get_many(self._h, xyz.ctypes.data, len(xyz), out.ctypes.data)
         return out
 
-    def cast(self, rays: np.ndarray, threads: int = 1, counters: bool = False):
-        """get_by_ray for each row (n x 6 f32: origin, direction)."""
+    def cast(self, rays: np.ndarray, threads: int = 1, counters: bool = False,
+             viewing_distance: float | None = None):
+        """get_by_ray for each row (n x 6 f32: origin, direction), or
+        get_by_ray_at_lod when a viewing distance is given."""
         rays = _f32(rays).reshape(-1, 6)
         out = np.zeros(len(rays), HIT_DTYPE)
         c = Counters()
-        lib().ora_cast(self._h, rays.ctypes.data, len(rays), out.ctypes.data, threads,
-                       C.byref(c) if counters else None)
+        if viewing_distance is None:
+            lib().ora_cast(self._h, rays.ctypes.data, len(rays), out.ctypes.data, threads,
+                           C.byref(c) if counters else None)
+        else:
+            lib().ora_cast_lod(self._h, rays.ctypes.data, len(rays), out.ctypes.data, threads,
+                               C.byref(c) if counters else None, float(viewing_distance))
         return (out, c) if counters else out
 
 

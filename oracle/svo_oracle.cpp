@@ -371,6 +371,10 @@ struct Octree {
   std::vector<uint32_t> voxels;
   std::vector<uint32_t> colors;  // Albedo packed r<<24|g<<16|b<<8|a
   std::vector<uint32_t> data;    // user data (u32)
+  // node_mips and MIPMapStrategy::enabled (types.rs:141-186)
+  std::vector<Brick> mips;
+  std::vector<uint32_t> mip_voxels;
+  bool mips_enabled = false;
 
   // ObjectPool::key_is_valid (object_pool.rs:238-240)
   bool key_is_valid(size_t key) const { return key < nodes.size() && nodes[key].reserved; }
@@ -420,6 +424,7 @@ struct Query {
   const Ray& ray;
   Counters* c;
   uint32_t flags = 0;  // bit 0: iteration guard, bit 1: BITMAP_INDEX_LUT index out of bounds
+  float viewing_distance = 3.40282347e+38f;  // f32::MAX: get_by_ray (:316-318)
 };
 
 const uint64_t MAX_STEPS = 1ull << 22;
@@ -545,8 +550,13 @@ static inline size_t bitmap_index(Query& q, V3f p) {
   return luts().bitmap_index[x][y][z];
 }
 
-// get_by_ray_at_lod (:325-565) with the MIP branch disabled (the default
-// MIPMapStrategy has enabled == false, mipmap.rs:591-604)
+// get_by_ray_at_lod (:325-565). The MIP branch (:368-386) runs only when the
+// tree's MIP strategy is enabled: at every stack-loop iteration the node's MIP
+// brick is probed over the node's bounds once the node's level falls below
+// the ray's travelled distance (to the point snapped to the level's grid)
+// divided by viewing_distance. mip_level starts at log2(size / dim), drops
+// by one per PUSH and rises by one per POP; it is not reset when the ray
+// restarts from the root.
 static bool get_by_ray(Query& q, Hit* hit) {
   const Octree& t = q.t;
   const Ray& ray = q.ray;
@@ -572,6 +582,7 @@ static bool get_by_ray(Query& q, Hit* hit) {
   }
   size_t current_node_key;
   uint64_t steps = 0;
+  float mip_level = std::log2(static_cast<float>(t.octree_size) / static_cast<float>(t.brick_dim));
   while (target_octant != OOB_OCTANT) {
     current_node_key = 0;  // ROOT_NODE_KEY (detail.rs:160)
     current_bounds = Cube::root_bounds(static_cast<float>(t.octree_size));
@@ -585,6 +596,17 @@ static bool get_by_ray(Query& q, Hit* hit) {
       const uint64_t current_node_occupied_bits = t.stored_occupied_bits(*node_stack.last());
       const Node& node = t.nodes[current_node_key];
       bool do_backtrack_after_leaf_miss = node.content == NodeContent::UniformLeaf;
+      if (t.mips_enabled) {
+        const float grid = mip_level * 2.f;
+        const V3f snapped = {std::round(ray_current_point.x / grid) * grid,
+                             std::round(ray_current_point.y / grid) * grid,
+                             std::round(ray_current_point.z / grid) * grid};
+        if (mip_level < length(sub(ray.origin, snapped)) / q.viewing_distance) {
+          static const Brick kEmptyMip{};
+          const Brick& mip = current_node_key < t.mips.size() ? t.mips[current_node_key] : kEmptyMip;
+          if (probe_brick(q, &ray_current_point, mip, current_bounds, ray_scale_factors, hit)) return true;
+        }
+      }
       if (target_octant != OOB_OCTANT) {
         if (node.content == NodeContent::UniformLeaf) {
           if (probe_brick(q, &ray_current_point, node.bricks[0], current_bounds, ray_scale_factors, hit)) return true;
@@ -608,6 +630,7 @@ static bool get_by_ray(Query& q, Hit* hit) {
           0 == (current_node_occupied_bits & L.ray_to_node_occupancy[flat_pos_in_bitmap][direction_lut_index])) {
         // POP
         node_stack.pop(nullptr);
+        mip_level += 1.f;
         if (uint32_t* parent = node_stack.last()) {
           current_node_key = *parent;
           const V3f current_bound_center = add(current_bounds.min_position, unit(current_bounds.size / 2.f));
@@ -634,6 +657,7 @@ static bool get_by_ray(Query& q, Hit* hit) {
         current_bounds = target_bounds;
         target_octant = hash_region(sub(ray_current_point, target_bounds.min_position), target_bounds.size / 2.f);
         node_stack.push(target_child_key);
+        mip_level -= 1.f;
       } else {
         // ADVANCE
         for (;;) {
@@ -751,6 +775,10 @@ struct ora_dump {
   const uint32_t* voxels;
   const uint32_t* colors;
   const uint32_t* data;
+  uint32_t mip_parted_count, mips_enabled;
+  const uint8_t* mip_kind;     // [pool_len]
+  const uint32_t* mip_value;   // [pool_len]: Solid value or Parted index
+  const uint32_t* mip_voxels;  // [mip_parted_count * dim^3]
 };
 
 struct ora_hit {
@@ -791,6 +819,18 @@ void* ora_tree_load(const ora_dump* d) {
                                              : nullptr;
     }
   }
+  t->mips_enabled = d->mips_enabled != 0;
+  if (d->mip_kind) {
+    t->mip_voxels.assign(d->mip_voxels, d->mip_voxels + static_cast<size_t>(d->mip_parted_count) * n3);
+    t->mips.resize(d->pool_len);
+    for (uint32_t k = 0; k < d->pool_len; ++k) {
+      Brick& b = t->mips[k];
+      b.kind = static_cast<BrickData>(d->mip_kind[k]);
+      b.solid = d->mip_value[k];
+      b.parted = b.kind == BrickData::Parted ? t->mip_voxels.data() + static_cast<size_t>(d->mip_value[k]) * n3
+                                             : nullptr;
+    }
+  }
   return t;
 }
 
@@ -805,10 +845,12 @@ void ora_get_many(const void* tp, const uint32_t* xyz, uint64_t n, uint32_t* out
   for (uint64_t i = 0; i < n; ++i) out[i] = ora::get(t, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
 }
 
-static void cast_one(const ora::Octree& t, const float* r, ora_hit* out, ora::Counters* c) {
+static void cast_one(const ora::Octree& t, const float* r, ora_hit* out, ora::Counters* c,
+                     float viewing_distance) {
   using namespace ora;
   const Ray ray = {{r[0], r[1], r[2]}, {r[3], r[4], r[5]}};
   Query q{t, ray, c};
+  q.viewing_distance = viewing_distance;
   Hit h{};
   const bool hit = get_by_ray(q, &h);
   out->hit = hit ? 1u : 0u;
@@ -844,12 +886,15 @@ static void cast_one(const ora::Octree& t, const float* r, ora_hit* out, ora::Co
 
 // get_by_ray for n rays {ox,oy,oz,dx,dy,dz}; `threads` > 1 splits the rays
 // into contiguous blocks over std::threads.
-void ora_cast(const void* tp, const float* rays, uint64_t n, ora_hit* out, uint32_t threads,
-              ora_counters* counters) {
+// get_by_ray_at_lod with the given viewing distance for n rays (f32::MAX is
+// get_by_ray); `threads` as for ora_cast.
+void ora_cast_lod(const void* tp, const float* rays, uint64_t n, ora_hit* out, uint32_t threads,
+                  ora_counters* counters, float viewing_distance) {
   const ora::Octree& t = *static_cast<const ora::Octree*>(tp);
   if (threads <= 1) {
     ora::Counters c;
-    for (uint64_t i = 0; i < n; ++i) cast_one(t, rays + 6 * i, out + i, counters ? &c : nullptr);
+    for (uint64_t i = 0; i < n; ++i)
+      cast_one(t, rays + 6 * i, out + i, counters ? &c : nullptr, viewing_distance);
     if (counters) {
       counters->rays += n;
       counters->stack_iterations += c.stack_iterations;
@@ -867,10 +912,15 @@ void ora_cast(const void* tp, const float* rays, uint64_t n, ora_hit* out, uint3
     const uint64_t b = k * chunk, e = std::min<uint64_t>(n, b + chunk);
     if (b >= e) break;
     pool.emplace_back([&, b, e]() {
-      for (uint64_t i = b; i < e; ++i) cast_one(t, rays + 6 * i, out + i, nullptr);
+      for (uint64_t i = b; i < e; ++i) cast_one(t, rays + 6 * i, out + i, nullptr, viewing_distance);
     });
   }
   for (auto& th : pool) th.join();
+}
+
+void ora_cast(const void* tp, const float* rays, uint64_t n, ora_hit* out, uint32_t threads,
+              ora_counters* counters) {
+  ora_cast_lod(tp, rays, n, out, threads, counters, 3.40282347e+38f);
 }
 
 // Primary rays of a w x h pinhole viewport {origin[3], direction[3],

@@ -4,7 +4,8 @@ The traversal hot path (Octree::get_by_ray) runs as hand-written HIP
 kernels for gfx950 inside libsvo.so, behind the C ABI of include/svo.h.
 """
 from .octree import (  # noqa: F401
-    COMPLEX, EMPTY, EMPTY_MARKER, INFORMATIVE, VISUAL, Batch, DeviceTree, FlatTree, Octree,
+    COMPLEX, EMPTY, EMPTY_MARKER, INFORMATIVE, VISUAL, Batch, DeviceTree, FlatTree,
+    MIPResamplingMethods, Octree, StrategyUpdater,
     OctreeEntry, OctreeError, Viewport, albedo, albedo_rgba, alloc_hits, batch_variants,
     collect_kernel_ms, current_batch_variant, current_render_variant, render_variants,
     select_batch_variant, select_render_variant, set_timing, voxel_data,
@@ -12,7 +13,7 @@ from .octree import (  # noqa: F401
 
 __all__ = [
     "COMPLEX", "EMPTY", "EMPTY_MARKER", "INFORMATIVE", "VISUAL", "Batch", "DeviceTree", "FlatTree",
-    "Octree", "OctreeEntry", "OctreeError", "Viewport", "albedo", "albedo_rgba", "alloc_hits",
+    "MIPResamplingMethods", "Octree", "StrategyUpdater", "OctreeEntry", "OctreeError", "Viewport", "albedo", "albedo_rgba", "alloc_hits",
     "batch_variants", "collect_kernel_ms", "current_batch_variant", "current_render_variant",
     "render_variants", "select_batch_variant", "select_render_variant", "set_timing",
     "voxel_data",

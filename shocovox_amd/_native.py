@@ -32,7 +32,8 @@ class Entry(C.Structure):
 class ModelSizes(C.Structure):
     _fields_ = [("octree_size", C.c_uint32), ("brick_dim", C.c_uint32), ("pool_len", C.c_uint32),
                 ("parted_count", C.c_uint32), ("color_count", C.c_uint32),
-                ("data_count", C.c_uint32)]
+                ("data_count", C.c_uint32), ("mip_parted_count", C.c_uint32),
+                ("mips_enabled", C.c_uint32)]
 
 
 class ModelDump(C.Structure):
@@ -40,7 +41,8 @@ class ModelDump(C.Structure):
                 ("internal_bits", C.c_void_p), ("children_kind", C.c_void_p),
                 ("children", C.c_void_p), ("children_bitmap", C.c_void_p),
                 ("brick_kind", C.c_void_p), ("brick_value", C.c_void_p), ("voxels", C.c_void_p),
-                ("colors", C.c_void_p), ("data", C.c_void_p)]
+                ("colors", C.c_void_p), ("data", C.c_void_p), ("mip_kind", C.c_void_p),
+                ("mip_value", C.c_void_p), ("mip_voxels", C.c_void_p)]
 
 
 class TreeDesc(C.Structure):
@@ -48,7 +50,7 @@ class TreeDesc(C.Structure):
                 ("node_count", C.c_uint32), ("brick_count", C.c_uint32),
                 ("color_count", C.c_uint32), ("top_level_nodes", C.c_uint32),
                 ("nodes", C.c_void_p), ("brick_pix", C.c_void_p), ("brick_occ", C.c_void_p),
-                ("colors", C.c_void_p)]
+                ("colors", C.c_void_p), ("mips_enabled", C.c_uint32)]
 
 
 class Hits(C.Structure):
@@ -115,6 +117,15 @@ _SIGS = {
     "svo_octree_clear": (C.c_int, [_P, _U32, _U32, _U32]),
     "svo_octree_clear_at_lod": (C.c_int, [_P, _U32, _U32, _U32, _U32]),
     "svo_octree_get": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Entry)]),
+    "svo_octree_switch_mip_maps": (C.c_int, [_P, C.c_int]),
+    "svo_octree_mip_maps_enabled": (C.c_int, [_P]),
+    "svo_octree_set_mip_method": (C.c_int, [_P, _U32, _U32, C.c_float]),
+    "svo_octree_get_mip_method": (C.c_int, [_P, _U32, C.POINTER(_U32), C.POINTER(C.c_float)]),
+    "svo_octree_set_mip_color_threshold": (C.c_int, [_P, _U32, C.c_float]),
+    "svo_octree_get_mip_color_threshold": (C.c_int, [_P, _U32, C.POINTER(C.c_float)]),
+    "svo_octree_reset_mip_strategy": (C.c_int, [_P]),
+    "svo_octree_recalculate_mips": (C.c_int, [_P]),
+    "svo_octree_sample_root_mip": (C.c_int, [_P, _U32, _U32, _U32, _U32, C.POINTER(Entry)]),
     "svo_octree_insert_many": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "svo_octree_clear_many": (C.c_int, [_P, _P, C.c_size_t]),
     "svo_octree_get_many": (C.c_int, [_P, _P, C.c_size_t, _P]),
@@ -141,6 +152,9 @@ _SIGS = {
     "svo_tree_set_step_limit": (C.c_int, [_P, _U32]),
     "svo_cast": (C.c_int, [_P, _P, _U32, C.POINTER(Hits), _P]),
     "svo_render": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.POINTER(Hits), _P]),
+    "svo_cast_at_lod": (C.c_int, [_P, _P, _U32, C.c_float, C.POINTER(Hits), _P]),
+    "svo_render_at_lod": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, C.c_float,
+                                    C.POINTER(Hits), _P]),
     "svo_render_rgba8": (C.c_int, [_P, C.POINTER(Viewport), _U32, _U32, _U32, _P, _P]),
     "svo_ray_occupancy_lut": (C.c_int, [_P]),
     "svo_batch_create": (C.c_int, [_U32, C.POINTER(_P)]),

@@ -293,8 +293,15 @@ bool octree_from_bencode(const uint8_t* data, size_t n, Octree* out, std::string
     kids.push_back(c);
   }
   if (!r.list_end()) return bail();
-  // node_mips: read and dropped
-  if (!r.skip()) return bail();
+  // node_mips
+  std::vector<Brick> mips;
+  if (!r.list_begin()) return bail();
+  while (r.ok() && !r.at_list_end()) {
+    Brick b;
+    if (!read_brick(r, n3, &b)) return bail();
+    mips.push_back(std::move(b));
+  }
+  if (!r.list_end()) return bail();
   // colour palette
   std::vector<uint32_t> colors;
   if (!r.list_begin()) return bail();
@@ -320,8 +327,46 @@ bool octree_from_bencode(const uint8_t* data, size_t n, Octree* out, std::string
     data_pal.push_back(v);
   }
   if (!r.list_end()) return bail();
-  // MIP strategy: read and dropped
-  if (!r.skip()) return bail();
+  // MIPMapStrategy (bytecode.rs:437-520): [enabled, n, (level, method)*n,
+  // m, (level, threshold * 1000)*m]; methods as bytecode.rs:522-570 encode them
+  MipStrategy strategy;
+  {
+    uint64_t enabled = 0, count = 0;
+    if (!r.list_begin() || !r.integer(&enabled) || !r.integer(&count)) return bail();
+    if (enabled > 1) {
+      r.fail("MIP enabled flag is not a boolean");
+      return bail();
+    }
+    strategy.enabled = enabled != 0;
+    strategy.methods.clear();
+    for (uint64_t i = 0; i < count; ++i) {
+      uint64_t level = 0, code = 0;
+      if (!r.integer(&level) || !r.integer(&code)) return bail();
+      MipMethodSpec m;
+      if (code <= 2) {
+        m.kind = static_cast<MipMethod>(code);
+      } else if (code >= 3 && code < 1002) {
+        m.kind = MipMethod::Posterize;
+        m.thr = (static_cast<float>(code) - 3.0f) / 1000.0f;
+      } else if (code >= 1003 && code < 2001) {
+        m.kind = MipMethod::PosterizeBD;
+        m.thr = (static_cast<float>(code) - 1003.0f) / 1000.0f;
+      } else {
+        r.fail("unknown MIP resampling method " + std::to_string(code));
+        return bail();
+      }
+      strategy.methods[static_cast<uint32_t>(level)] = m;
+    }
+    if (!r.integer(&count)) return bail();
+    strategy.color_thresholds.clear();
+    for (uint64_t i = 0; i < count; ++i) {
+      uint64_t level = 0;
+      uint32_t thr = 0;
+      if (!r.integer(&level) || !r.integer32(&thr)) return bail();
+      strategy.color_thresholds[static_cast<uint32_t>(level)] = static_cast<float>(thr) / 1000.0f;
+    }
+    if (!r.list_end()) return bail();
+  }
   if (!r.list_end()) return bail();
   if (kids.size() != items.size()) {
     *err = "node and children counts differ (" + std::to_string(items.size()) + " vs " +
@@ -342,6 +387,7 @@ bool octree_from_bencode(const uint8_t* data, size_t n, Octree* out, std::string
   if (first_available > items.size()) first_available = items.size();
   t.assign_model(std::move(items), std::move(reserved), first_available, std::move(kids),
                  std::move(colors), std::move(data_pal), auto_simplify != 0);
+  t.assign_mips(std::move(mips), std::move(strategy));
   *out = std::move(t);
   return true;
 }
@@ -404,8 +450,8 @@ void octree_to_bencode(const Octree& t, std::string* out) {
     }
   }
   o.push_back('e');
-  o.push_back('l');  // node_mips: one empty brick per node
-  for (uint32_t k = 0; k < pool.len(); ++k) put_str(&o, "#b");
+  o.push_back('l');  // node_mips: one brick per node
+  for (uint32_t k = 0; k < pool.len(); ++k) put_brick(&o, t.node_mip(k));
   o.push_back('e');
   o.push_back('l');
   for (uint32_t c : t.color_palette()) {
@@ -420,7 +466,27 @@ void octree_to_bencode(const Octree& t, std::string* out) {
   o.push_back('l');
   for (uint32_t v : t.data_palette()) put_int(&o, v);
   o.push_back('e');
-  o.append("li0ei0ei0ee");  // MIP strategy: disabled, no methods, no thresholds
+  // MIP strategy, levels in ascending order (the reference writes its
+  // HashMaps in their iteration order)
+  const MipStrategy& ms = t.mip_strategy();
+  o.push_back('l');
+  put_int(&o, ms.enabled ? 1 : 0);
+  put_int(&o, ms.methods.size());
+  for (const auto& kv : ms.methods) {
+    put_int(&o, kv.first);
+    const MipMethodSpec& m = kv.second;
+    uint64_t code = static_cast<uint64_t>(m.kind);
+    // (threshold * 1000.) as u32: truncation of the f32 product
+    if (m.kind == MipMethod::Posterize) code = 3 + f2u32(m.thr * 1000.0f);
+    if (m.kind == MipMethod::PosterizeBD) code = 1003 + f2u32(m.thr * 1000.0f);
+    put_int(&o, code);
+  }
+  put_int(&o, ms.color_thresholds.size());
+  for (const auto& kv : ms.color_thresholds) {
+    put_int(&o, kv.first);
+    put_int(&o, f2u32(kv.second * 1000.0f));
+  }
+  o.push_back('e');
   o.push_back('e');
 }
 

@@ -9,13 +9,13 @@
 
 namespace svo {
 
-// Parses a serialized octree (Octree::from_bytes, octree/mod.rs:143-147).
-// MIP maps and the MIP strategy are read and dropped: the traversal here runs
-// with MIPs disabled. Returns false with a message on malformed input.
+// Parses a serialized octree (Octree::from_bytes, octree/mod.rs:143-147),
+// its MIP maps and MIP strategy included. Returns false with a message on
+// malformed input.
 bool octree_from_bencode(const uint8_t* data, size_t n, Octree* out, std::string* err);
 
-// Serializes an octree as Octree::to_bytes (octree/mod.rs:136-140) does; node
-// MIPs are written as empty bricks and the MIP strategy as disabled.
+// Serializes an octree as Octree::to_bytes (octree/mod.rs:136-140) does; the
+// strategy's levels are written in ascending order.
 void octree_to_bencode(const Octree& t, std::string* out);
 
 }  // namespace svo

@@ -80,6 +80,7 @@ void fill_desc(const svo::FlatTree& f, svo_tree_desc* d) {
   d->brick_pix = f.brick_pix.data();
   d->brick_occ = f.brick_occ.data();
   d->colors = f.colors.data();
+  d->mips_enabled = f.mips_enabled ? 1u : 0u;
 }
 
 }  // namespace
@@ -149,6 +150,68 @@ int svo_octree_get(const svo_octree* tree, uint32_t x, uint32_t y, uint32_t z, s
   return SVO_OK;
 }
 
+int svo_octree_switch_mip_maps(svo_octree* tree, int enabled) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  tree->tree.switch_albedo_mip_maps(enabled != 0);
+  return SVO_OK;
+}
+
+int svo_octree_mip_maps_enabled(const svo_octree* tree) {
+  return tree && tree->tree.mip_strategy().enabled ? 1 : 0;
+}
+
+int svo_octree_set_mip_method(svo_octree* tree, uint32_t level, uint32_t method, float thr) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  if (method > SVO_MIP_POSTERIZE_BD) return fail(SVO_INVALID_ARGUMENT, "unknown MIP method");
+  tree->tree.set_mip_method(level, svo::MipMethodSpec{static_cast<svo::MipMethod>(method), thr});
+  return SVO_OK;
+}
+
+int svo_octree_get_mip_method(const svo_octree* tree, uint32_t level, uint32_t* method,
+                              float* thr) {
+  if (!tree || !method || !thr) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const svo::MipMethodSpec m = tree->tree.mip_strategy().method_at(level);
+  *method = static_cast<uint32_t>(m.kind);
+  *thr = m.thr;
+  return SVO_OK;
+}
+
+int svo_octree_set_mip_color_threshold(svo_octree* tree, uint32_t level, float thr) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  tree->tree.set_mip_color_threshold(level, thr);
+  return SVO_OK;
+}
+
+int svo_octree_get_mip_color_threshold(const svo_octree* tree, uint32_t level, float* thr) {
+  if (!tree || !thr) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const auto& m = tree->tree.mip_strategy().color_thresholds;
+  auto it = m.find(level);
+  *thr = it == m.end() ? 0.0f : it->second;
+  return SVO_OK;
+}
+
+int svo_octree_reset_mip_strategy(svo_octree* tree) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  tree->tree.reset_mip_strategy();
+  return SVO_OK;
+}
+
+int svo_octree_recalculate_mips(svo_octree* tree) {
+  if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
+  tree->tree.recalculate_mips();
+  return SVO_OK;
+}
+
+int svo_octree_sample_root_mip(const svo_octree* tree, uint32_t octant, uint32_t x, uint32_t y,
+                               uint32_t z, svo_entry* out) {
+  if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  const uint32_t d = tree->tree.brick_dim();
+  if (octant > 8 || x >= d || y >= d || z >= d)
+    return fail(SVO_INVALID_ARGUMENT, "octant above 8 or position outside the brick");
+  from_entry(tree->tree.sample_root_mip(octant, x, y, z), out);
+  return SVO_OK;
+}
+
 int svo_octree_insert_many(svo_octree* tree, const uint32_t* xyz, const svo_entry* entries,
                            size_t n) {
   if (!tree || (n && (!xyz || !entries))) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
@@ -198,7 +261,24 @@ int svo_octree_export(const svo_octree* tree, svo_model_sizes* sizes, svo_model_
   sizes->parted_count = parted;
   sizes->color_count = static_cast<uint32_t>(t.color_palette().size());
   sizes->data_count = static_cast<uint32_t>(t.data_palette().size());
+  uint32_t mip_parted = 0;
+  for (uint32_t k = 0; k < pool.len(); ++k)
+    if (t.node_mip(k).kind == svo::BrickKind::Parted) ++mip_parted;
+  sizes->mip_parted_count = mip_parted;
+  sizes->mips_enabled = t.mip_strategy().enabled ? 1u : 0u;
   if (!dump) return SVO_OK;
+  uint32_t next_mip = 0;
+  for (uint32_t k = 0; k < pool.len(); ++k) {
+    const svo::Brick& m = t.node_mip(k);
+    dump->mip_kind[k] = static_cast<uint8_t>(m.kind);
+    dump->mip_value[k] = m.kind == svo::BrickKind::Solid ? m.solid : 0u;
+    if (m.kind == svo::BrickKind::Parted) {
+      dump->mip_value[k] = next_mip;
+      std::memcpy(dump->mip_voxels + static_cast<size_t>(next_mip) * n3, m.parted.data(),
+                  n3 * sizeof(uint32_t));
+      ++next_mip;
+    }
+  }
   uint32_t next_parted = 0;
   for (uint32_t k = 0; k < pool.len(); ++k) {
     const svo::NodeContent& n = pool.get(k);
@@ -487,6 +567,31 @@ int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32
       vp->frustum[1], vp->fov, w, h);
   std::string err;
   const int st = svo::device_render(*dtree->dev, cam, w, h, *out, stream, &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
+int svo_cast_at_lod(const svo_tree* dtree, const float* rays, uint32_t n,
+                    float viewing_distance, const svo_hits* out, void* stream) {
+  if (!dtree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  if (!(viewing_distance > 0.0f)) return fail(SVO_INVALID_ARGUMENT, "viewing distance must be > 0");
+  std::string err;
+  const int st = svo::device_cast_lod(*dtree->dev, rays, n, viewing_distance, *out, stream, &err);
+  if (st != SVO_OK) return fail(st, err);
+  return SVO_OK;
+}
+
+int svo_render_at_lod(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
+                      float viewing_distance, const svo_hits* out, void* stream) {
+  if (!dtree || !vp || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  if (!(viewing_distance > 0.0f)) return fail(SVO_INVALID_ARGUMENT, "viewing distance must be > 0");
+  const svo::Camera cam = svo::make_camera(
+      svo::V3f{vp->origin[0], vp->origin[1], vp->origin[2]},
+      svo::V3f{vp->direction[0], vp->direction[1], vp->direction[2]}, vp->frustum[0],
+      vp->frustum[1], vp->fov, w, h);
+  std::string err;
+  const int st =
+      svo::device_render_lod(*dtree->dev, cam, w, h, viewing_distance, *out, stream, &err);
   if (st != SVO_OK) return fail(st, err);
   return SVO_OK;
 }

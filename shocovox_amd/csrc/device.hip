@@ -31,6 +31,35 @@ __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __re
   store_result(t, out, i, res, o);
 }
 
+// get_by_ray_at_lod for a ray array (MIP bricks probed per the viewing
+// distance when the tree's MIPs are enabled).
+__global__ __launch_bounds__(256) void cast_lod_kernel(TreeView t, const float* __restrict__ rays,
+                                                       uint32_t n, float viewing_distance,
+                                                       HitPtrs out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rays + 6ull * i;
+  const V3f o{r[0], r[1], r[2]};
+  const V3f d{r[3], r[4], r[5]};
+  const CastResult res = cast_ray_lod(t, t.ray_lut, o, d, viewing_distance);
+  store_result(t, out, i, res, o);
+}
+
+// get_by_ray_at_lod for the primary rays of a viewport: four 64-lane waves
+// per workgroup, each over one 8x8 pixel tile.
+__global__ __launch_bounds__(256) void render_lod_kernel(TreeView t, Camera cam, uint32_t w,
+                                                         uint32_t h, float viewing_distance,
+                                                         HitPtrs out) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t x = (blockIdx.x * 2u + (wave & 1u)) * 8u + (lane % 8u);
+  const uint32_t y = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane / 8u);
+  if (x >= w || y >= h) return;
+  const V3f d = camera_ray_dir(cam, x, y);
+  const CastResult res = cast_ray_lod(t, t.ray_lut, cam.origin, d, viewing_distance);
+  store_result(t, out, y * w + x, res, cam.origin);
+}
+
 // Primary-ray render. A workgroup is four 64-lane waves arranged 2x2; each
 // wave covers a TW x TH pixel tile (TW * TH = 64). MIN_WAVES asks the
 // compiler for at least that many waves per SIMD (it caps VGPRs to fit);
@@ -284,6 +313,10 @@ bool validate_desc(const svo_tree_desc& d, std::string* err) {
   for (uint32_t i = 0; i < d.node_count; ++i) {
     const svo_node_rec& r = d.nodes[i];
     const uint32_t kind = r.meta & 3u;
+    if (r.mip > FORM_PARTED || (r.mip == FORM_PARTED && r.mip_word >= d.brick_count)) {
+      *err = "MIP brick reference out of range";
+      return false;
+    }
     if (kind == KIND_INTERNAL) {
       for (uint32_t o = 0; o < 8; ++o)
         if (r.child[o] != INVALID_INDEX && r.child[o] >= d.node_count) {
@@ -440,6 +473,7 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   t->view.inv_dim = pow2_rcp(static_cast<float>(d.brick_dim));
   t->view.node_count = d.node_count;
   t->view.max_steps = t->step_limit ? t->step_limit : MAX_TRAVERSAL_STEPS;
+  t->view.mips_enabled = d.mips_enabled;
   return SVO_OK;
 }
 
@@ -661,6 +695,7 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   t->color_count = d.color_count;
   t->view.color_count = d.color_count;
   t->view.node_count = d.node_count;
+  t->view.mips_enabled = d.mips_enabled;
   return SVO_OK;
 }
 
@@ -693,6 +728,39 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
   hipLaunchKernelGGL(cast_kernel, dim3(grid), dim3(block), 0, s, t.view, rays, n, to_ptrs(out));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "cast_kernel launch", err);
+  timer.done();
+  return SVO_OK;
+}
+
+int device_cast_lod(const DeviceTree& t, const float* rays, uint32_t n, float viewing_distance,
+                    const svo_hits& out, void* stream, std::string* err) {
+  if (n == 0) return SVO_OK;
+  if (rays == nullptr) {
+    *err = "rays is NULL";
+    return SVO_INVALID_ARGUMENT;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  LaunchTimer timer(s);
+  const uint32_t grid = (n + 255u) / 256u;
+  hipLaunchKernelGGL(cast_lod_kernel, dim3(grid), dim3(256), 0, s, t.view, rays, n,
+                     viewing_distance, to_ptrs(out));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "cast_lod_kernel launch", err);
+  timer.done();
+  return SVO_OK;
+}
+
+int device_render_lod(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
+                      float viewing_distance, const svo_hits& out, void* stream,
+                      std::string* err) {
+  if (w == 0 || h == 0) return SVO_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  LaunchTimer timer(s);
+  const dim3 grid((w + 15u) / 16u, (h + 15u) / 16u);
+  hipLaunchKernelGGL(render_lod_kernel, grid, dim3(256), 0, s, t.view, cam, w, h,
+                     viewing_distance, to_ptrs(out));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "render_lod_kernel launch", err);
   timer.done();
   return SVO_OK;
 }

@@ -61,6 +61,12 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
                 void* stream, std::string* err);
 int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
                   const svo_hits& out, void* stream, std::string* err);
+// get_by_ray_at_lod over a ray array / a viewport's primary rays
+int device_cast_lod(const DeviceTree& t, const float* rays, uint32_t n, float viewing_distance,
+                    const svo_hits& out, void* stream, std::string* err);
+int device_render_lod(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
+                      float viewing_distance, const svo_hits& out, void* stream,
+                      std::string* err);
 // Shaded RGBA8 render (conventions described at shade_kernel).
 int device_render_rgba8(const DeviceTree& t, V3f origin, V3f dir, float glass_w, float glass_h,
                         float fov, uint32_t w, uint32_t h, uint32_t convention, uint32_t* out,

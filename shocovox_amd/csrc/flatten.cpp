@@ -47,6 +47,7 @@ FlatTree flatten(const Octree& tree, bool headroom) {
   f.octree_size = tree.size();
   f.brick_dim = tree.brick_dim();
   f.colors = tree.color_palette();
+  f.mips_enabled = tree.mip_strategy().enabled;
   const uint32_t dim = tree.brick_dim();
   const size_t n3 = static_cast<size_t>(dim) * dim * dim;
   const uint32_t words = occ_words_per_brick(dim);
@@ -104,6 +105,16 @@ FlatTree flatten(const Octree& tree, bool headroom) {
     r = NodeRec{};
     for (auto& c : r.child) c = INVALID_INDEX;
     r.occ = tree.stored_occupied_bits(key);
+    // the node's MIP brick (its Parted bricks share the brick slots)
+    const Brick& mip = tree.node_mip(key);
+    if (mip.kind == BrickKind::Solid) {
+      r.mip = FORM_SOLID;
+      r.mip_word = mip.solid;
+    } else if (mip.kind == BrickKind::Parted) {
+      r.mip = FORM_PARTED;
+      r.mip_word = static_cast<uint32_t>(parted.size());
+      parted.push_back(&mip.parted);
+    }
     switch (n.kind) {
       case NodeKind::Nothing:
         r.meta = KIND_NOTHING;
@@ -283,6 +294,9 @@ void LayoutMirror::release_key(uint32_t key) {
   if (kind == KIND_LEAF || kind == KIND_UNIFORM_LEAF)
     for (uint32_t o = 0; o < 8; ++o)
       if (brick_form(rec.meta, o) == FORM_PARTED) release_slot(rec.child[o]);
+  if (rec.mip == FORM_PARTED) release_slot(rec.mip_word);
+  rec.mip = FORM_EMPTY;
+  rec.mip_word = 0;
   rec.meta = KIND_NOTHING;
   rec.occ = 0;
   key_to_rec_[key] = NO_RECORD;
@@ -332,6 +346,17 @@ void LayoutMirror::write_record(const Octree& tree, uint32_t key, std::vector<ui
   // slots the record held before, per octant
   uint32_t old_slots[8];
   const uint32_t old_kind = node_kind(before.meta);
+  uint32_t old_mip_slot = before.mip == FORM_PARTED ? before.mip_word : NO_RECORD;
+  const Brick& mip = tree.node_mip(key);
+  if (mip.kind == BrickKind::Solid) {
+    rec.mip = FORM_SOLID;
+    rec.mip_word = mip.solid;
+  } else if (mip.kind == BrickKind::Parted) {
+    rec.mip = FORM_PARTED;
+    rec.mip_word = write_brick(tree, mip, old_mip_slot);
+    old_mip_slot = NO_RECORD;
+  }
+  if (old_mip_slot != NO_RECORD) release_slot(old_mip_slot);
   for (uint32_t o = 0; o < 8; ++o)
     old_slots[o] = ((old_kind == KIND_LEAF || old_kind == KIND_UNIFORM_LEAF) &&
                     brick_form(before.meta, o) == FORM_PARTED)
@@ -384,6 +409,7 @@ void LayoutMirror::write_record(const Octree& tree, uint32_t key, std::vector<ui
 void LayoutMirror::apply(Octree& tree) {
   std::vector<uint32_t> keys = tree.take_dirty();
   flat_.colors = tree.color_palette();
+  flat_.mips_enabled = tree.mip_strategy().enabled;
   // released nodes first, so their records and slots can be reused
   for (uint32_t k : keys)
     if (!tree.nodes().key_is_valid(k)) release_key(k);

@@ -17,6 +17,7 @@ struct FlatTree {
   std::vector<uint32_t> brick_occ;  // brick_count * occ_words_per_brick(dim)
   std::vector<uint32_t> colors;     // packed albedo, r<<24|g<<16|b<<8|a
   std::vector<uint32_t> level_end;  // level_end[l] = first record index of level l+1
+  bool mips_enabled = false;
   uint32_t brick_count() const {
     const uint64_t n = static_cast<uint64_t>(brick_dim) * brick_dim * brick_dim;
     return n ? static_cast<uint32_t>(brick_pix.size() / n) : 0;

@@ -27,12 +27,15 @@ constexpr uint32_t INVALID_INDEX = 0xFFFFFFFFu;
 enum : uint32_t { KIND_NOTHING = 0, KIND_INTERNAL = 1, KIND_LEAF = 2, KIND_UNIFORM_LEAF = 3 };
 enum : uint32_t { FORM_EMPTY = 0, FORM_SOLID = 1, FORM_PARTED = 2 };
 
+// A node's MIP brick (node_mips[key]) rides in `mip` (its FORM_*) and
+// `mip_word` (the Solid value or the Parted brick slot).
 struct alignas(64) NodeRec {
   uint64_t occ;
   uint32_t meta;
-  uint32_t reserved0;
+  uint32_t mip;
   uint32_t child[8];
-  uint32_t reserved1[4];
+  uint32_t mip_word;
+  uint32_t reserved[3];
 };
 static_assert(sizeof(NodeRec) == 64, "node record must be 64 bytes");
 

@@ -320,10 +320,13 @@ Entry Octree::pix_get_ref(uint32_t pix) const {
 
 Entry Octree::get(uint32_t x, uint32_t y, uint32_t z) const {
   const uint32_t pos[3] = {x, y, z};
+  return get_internal(ROOT_NODE_KEY, Cube{v3u(0.0f), static_cast<float>(octree_size_)}, pos);
+}
+
+// get_internal (octree/mod.rs:219-371): the point query from a given node
+Entry Octree::get_internal(uint32_t key, Cube bounds, const uint32_t pos[3]) const {
   const V3f p = to_f(pos);
-  Cube bounds{v3u(0.0f), static_cast<float>(octree_size_)};
   if (!bound_contains(bounds, p)) return Entry{};
-  uint32_t key = ROOT_NODE_KEY;
   const uint32_t dim = brick_dim_;
   for (;;) {
     const NodeContent& n = nodes_.get(key);
@@ -1100,6 +1103,7 @@ Status Octree::insert_at_lod_internal(bool overwrite_if_empty, uint32_t x, uint3
       set_occupancy_in_bitmap_64bits(p, actual_update_size, f2usize(nb.size), true, &bits);
     }
     store_occupied_bits(key, bits);
+    update_mip(key, nb, rpos);
     if (is_leaf(nodes_.get(key).kind)) {
       simplifyable = simplify(key);
       continue;
@@ -1224,10 +1228,377 @@ Status Octree::clear_at_lod(uint32_t x, uint32_t y, uint32_t z, uint32_t clear_s
       kids(key) = NodeChildren{};
     else
       store_occupied_bits(key, bits);
+    update_mip(key, nb, pos);
     if (simplifyable) simplifyable = simplify(key);
     if (prev == bits) break;
   }
   return Status::Ok;
+}
+
+}  // namespace svo
+
+// ---------------------------------------------------------------------------
+// MIP maps (mipmap.rs)
+
+namespace svo {
+
+namespace {
+
+struct Rgba {
+  uint32_t r, g, b, a;
+};
+inline Rgba unpack(uint32_t c) { return Rgba{c >> 24, (c >> 16) & 0xFFu, (c >> 8) & 0xFFu, c & 0xFFu}; }
+inline uint32_t pack(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+  return (r << 24) | (g << 16) | (b << 8) | a;
+}
+// Rust's `f32 as u8`: truncation toward zero, saturating, NaN to 0
+inline uint32_t f2u8(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 255.0f) return 255u;
+  return static_cast<uint32_t>(v);
+}
+
+// Albedou32 (mipmap.rs:35-120) with the wrapping u32 arithmetic of a release
+// build: differences below zero wrap, as do squares past 2^32.
+struct Acc {
+  uint32_t r, g, b, a;
+};
+inline Acc acc_of(Rgba c) { return Acc{c.r, c.g, c.b, c.a}; }
+inline Acc acc_pow2(Acc v) { return Acc{v.r * v.r, v.g * v.g, v.b * v.b, v.a * v.a}; }
+inline Acc acc_add(Acc x, Acc y) { return Acc{x.r + y.r, x.g + y.g, x.b + y.b, x.a + y.a}; }
+inline Acc acc_sub(Acc x, Acc y) { return Acc{x.r - y.r, x.g - y.g, x.b - y.b, x.a - y.a}; }
+inline uint32_t round_f2u32(float v) { return f2u32(roundf(v)); }
+inline Acc acc_div(Acc x, uint32_t d) {
+  const float fd = static_cast<float>(d);
+  return Acc{round_f2u32(static_cast<float>(x.r) / fd), round_f2u32(static_cast<float>(x.g) / fd),
+             round_f2u32(static_cast<float>(x.b) / fd), round_f2u32(static_cast<float>(x.a) / fd)};
+}
+inline Acc acc_sqrt(Acc x) {
+  return Acc{round_f2u32(sqrtf(static_cast<float>(x.r))), round_f2u32(sqrtf(static_cast<float>(x.g))),
+             round_f2u32(sqrtf(static_cast<float>(x.b))), round_f2u32(sqrtf(static_cast<float>(x.a)))};
+}
+inline float acc_length(Acc x) {
+  return sqrtf(static_cast<float>(x.r * x.r + x.g * x.g + x.b * x.b + x.a * x.a));
+}
+inline uint32_t acc_to_albedo(Acc x) {
+  return pack(std::min(x.r, 255u), std::min(x.g, 255u), std::min(x.b, 255u), std::min(x.a, 255u));
+}
+
+// Albedo::distance_from (detail.rs:82-89)
+inline float color_distance(uint32_t x, uint32_t y) {
+  const Rgba p = unpack(x), q = unpack(y);
+  const float dr = static_cast<float>(p.r) - static_cast<float>(q.r);
+  const float dg = static_cast<float>(p.g) - static_cast<float>(q.g);
+  const float db = static_cast<float>(p.b) - static_cast<float>(q.b);
+  const float da = static_cast<float>(p.a) - static_cast<float>(q.a);
+  return sqrtf(dr * dr + dg * dg + db * db + da * da);
+}
+
+// MIPResaplingFunction::execute (mipmap.rs:132-269). `sample(x, y, z, &rgba)`
+// reports whether the position holds a colour. Where the reference picks
+// among equals through a HashMap (PointFilter's most frequent colour,
+// Posterize's first similar group and largest group), whose iteration order
+// changes from run to run, the colour met first in sampling order is taken.
+template <class F>
+bool resample(MipMethodSpec m, const uint32_t start[3], uint32_t size, F sample, uint32_t* out) {
+  if (m.kind == MipMethod::BoxFilter) {
+    bool any = false;
+    uint32_t count = 0;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t x = start[0]; x < start[0] + size; ++x)
+      for (uint32_t y = start[1]; y < start[1] + size; ++y)
+        for (uint32_t z = start[2]; z < start[2] + size; ++z) {
+          uint32_t c;
+          if (!sample(x, y, z, &c)) continue;
+          const Rgba v = unpack(c);
+          const float q[4] = {static_cast<float>(v.r), static_cast<float>(v.g),
+                              static_cast<float>(v.b), static_cast<float>(v.a)};
+          if (!any) {
+            any = true;
+            count = 1;
+            for (int k = 0; k < 4; ++k) s[k] = q[k] * q[k];
+          } else {
+            ++count;
+            for (int k = 0; k < 4; ++k) s[k] += q[k] * q[k];
+          }
+        }
+    if (!any) return false;
+    const float n = static_cast<float>(count);
+    uint32_t o[4];
+    for (int k = 0; k < 4; ++k) o[k] = f2u8(fminf(sqrtf(s[k] / n), 255.0f));
+    *out = pack(o[0], o[1], o[2], o[3]);
+    return true;
+  }
+  if (m.kind == MipMethod::PointFilter || m.kind == MipMethod::PointFilterBD) {
+    std::vector<std::pair<uint32_t, uint32_t>> counts;  // colour, occurrences
+    for (uint32_t x = start[0]; x < start[0] + size; ++x)
+      for (uint32_t y = start[1]; y < start[1] + size; ++y)
+        for (uint32_t z = start[2]; z < start[2] + size; ++z) {
+          uint32_t c;
+          if (!sample(x, y, z, &c)) continue;
+          bool found = false;
+          for (auto& e : counts)
+            if (e.first == c) {
+              ++e.second;
+              found = true;
+              break;
+            }
+          if (!found) counts.emplace_back(c, 1u);
+        }
+    if (counts.empty()) return false;
+    size_t best = 0;
+    for (size_t i = 1; i < counts.size(); ++i)
+      if (counts[i].second > counts[best].second) best = i;
+    *out = counts[best].first;
+    return true;
+  }
+  // Posterize / PosterizeBD: groups of similar colours as (sum of squares, count)
+  std::vector<std::pair<Acc, uint32_t>> groups;
+  const float thr = m.thr * 255.0f;
+  for (uint32_t x = start[0]; x < start[0] + size; ++x)
+    for (uint32_t y = start[1]; y < start[1] + size; ++y)
+      for (uint32_t z = start[2]; z < start[2] + size; ++z) {
+        uint32_t c;
+        if (!sample(x, y, z, &c)) continue;
+        const Acc col = acc_of(unpack(c));
+        bool merged = false;
+        for (auto& g : groups) {
+          const Acc poster = acc_sqrt(acc_div(g.first, g.second));
+          if (acc_length(acc_sub(poster, col)) < thr) {
+            g.first = acc_add(g.first, acc_pow2(col));
+            ++g.second;
+            merged = true;
+            break;
+          }
+        }
+        if (!merged) groups.emplace_back(acc_pow2(col), 1u);
+      }
+  if (groups.empty()) return false;
+  size_t best = 0;
+  for (size_t i = 1; i < groups.size(); ++i)
+    if (groups[i].second > groups[best].second) best = i;
+  *out = acc_to_albedo(acc_sqrt(acc_div(groups[best].first, groups[best].second)));
+  return true;
+}
+
+inline bool albedo_of(const Entry& e, uint32_t* out) {
+  if (e.kind != EntryKind::Visual && e.kind != EntryKind::Complex) return false;
+  *out = e.albedo;
+  return true;
+}
+
+inline uint32_t log2_level(float ratio) {
+  // (node size / brick dim).log2() as usize: truncation of an exact power of two
+  return f2u32(log2f(ratio));
+}
+
+}  // namespace
+
+const Brick& Octree::node_mip(uint32_t key) const {
+  static const Brick kEmpty{};
+  return key < node_mips_.size() ? node_mips_[key] : kEmpty;
+}
+
+Brick& Octree::mip_mut(uint32_t key) {
+  if (key >= node_mips_.size()) node_mips_.resize(static_cast<size_t>(key) + 1);
+  nodes_.touch(key);
+  return node_mips_[key];
+}
+
+void Octree::set_mip_method(uint32_t level, MipMethodSpec m) {
+  // set_method_at_internal (mipmap.rs:657-674)
+  if (m.kind == MipMethod::Posterize || m.kind == MipMethod::PosterizeBD)
+    m.thr = std::min(std::max(m.thr, 0.0f), 1.0f);
+  mip_strategy_.methods[level] = m;
+}
+
+void Octree::set_mip_color_threshold(uint32_t level, float thr) {
+  // set_color_similarity_thr_internal (mipmap.rs:618-631)
+  mip_strategy_.color_thresholds[level] = std::min(std::max(thr, 0.0f), 1.0f);
+}
+
+void Octree::reset_mip_strategy() { mip_strategy_ = MipStrategy{}; }
+
+void Octree::assign_mips(std::vector<Brick> mips, MipStrategy strategy) {
+  node_mips_ = std::move(mips);
+  if (node_mips_.empty()) node_mips_.resize(1);
+  mip_strategy_ = std::move(strategy);
+  for (uint32_t k = 0; k < node_mips_.size() && k < nodes_.len(); ++k) nodes_.touch(k);
+  ++edit_epoch_;
+}
+
+void Octree::switch_albedo_mip_maps(bool enabled) {
+  const bool before = mip_strategy_.enabled;
+  mip_strategy_.enabled = enabled;
+  if (enabled && before != enabled && nodes_.get(ROOT_NODE_KEY).kind != NodeKind::Nothing)
+    recalculate_mips();
+  ++edit_epoch_;
+}
+
+// recalculate_mips (mipmap.rs:798-862): depth first, children before parents
+void Octree::recalculate_mips() {
+  node_mips_.assign(nodes_.len() ? nodes_.len() : 1, Brick{});
+  for (uint32_t k = 0; k < nodes_.len(); ++k) nodes_.touch(k);
+  struct Item {
+    uint32_t key;
+    Cube bounds;
+    uint32_t octant;
+  };
+  std::vector<Item> stack{{ROOT_NODE_KEY, Cube{v3u(0.0f), static_cast<float>(octree_size_)}, 0}};
+  while (!stack.empty()) {
+    Item& it = stack.back();
+    if (it.octant == OOB_OCTANT) {
+      recalculate_mip(it.key, it.bounds);
+      stack.pop_back();
+      if (!stack.empty()) stack.back().octant += 1;
+      continue;
+    }
+    const NodeContent& n = nodes_.get(it.key);
+    if (n.kind == NodeKind::Internal) {
+      const uint32_t c = node_children_[it.key].child(it.octant);
+      if (nodes_.key_is_valid(c) && nodes_.get(c).kind != NodeKind::Nothing) {
+        const Item next{c, child_bounds_for(it.bounds, it.octant), 0};
+        stack.push_back(next);
+      } else {
+        it.octant += 1;
+      }
+    } else {
+      // Nothing cannot be reached (the root is checked by the caller and
+      // empty children are skipped); leaves are evaluated at once
+      it.octant = OOB_OCTANT;
+    }
+  }
+}
+
+// recalculate_mip (mipmap.rs:882-902)
+void Octree::recalculate_mip(uint32_t key, Cube bounds) {
+  if (!mip_strategy_.enabled) return;
+  const float dim = static_cast<float>(brick_dim_);
+  for (uint32_t x = 0; x < brick_dim_; ++x)
+    for (uint32_t y = 0; y < brick_dim_; ++y)
+      for (uint32_t z = 0; z < brick_dim_; ++z) {
+        const V3f v{static_cast<float>(x), static_cast<float>(y), static_cast<float>(z)};
+        const V3f scaled = (v * bounds.size) / dim;
+        const V3f p = bounds.min + V3f{roundf(scaled.x), roundf(scaled.y), roundf(scaled.z)};
+        const uint32_t pos[3] = {round_u32(p.x), round_u32(p.y), round_u32(p.z)};
+        update_mip(key, bounds, pos);
+      }
+}
+
+// update_mip (mipmap.rs:297-589)
+void Octree::update_mip(uint32_t key, Cube bounds, const uint32_t pos[3]) {
+  if (!mip_strategy_.enabled) return;
+  const uint32_t level = log2_level(bounds.size / static_cast<float>(brick_dim_));
+  const MipMethodSpec sampler = mip_strategy_.method_at(level);
+  const bool bottom = sampler.kind == MipMethod::PointFilterBD;
+  const NodeContent& n = nodes_.get(key);
+  const uint32_t dim = brick_dim_;
+  uint32_t start[3];
+  uint32_t size = 0;
+  auto floor_u32 = [](float v) { return round_u32(floorf(v)); };
+  switch (n.kind) {
+    case NodeKind::Nothing:
+      return;
+    case NodeKind::UniformLeaf:
+      // a uniform leaf is its own MIP
+      if (node_mip(key).kind != BrickKind::Empty) mip_mut(key) = Brick{};
+      return;
+    case NodeKind::Leaf: {
+      size = std::min(f2u32(bounds.size) / dim, dim * 2u);
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t aligned = (pos[k] - pos[k] % size) * 2u * dim;
+        start[k] = floor_u32(static_cast<float>(aligned) / bounds.size);
+      }
+      break;
+    }
+    case NodeKind::Internal:
+      if (bottom) {
+        size = f2u32(bounds.size) / dim;
+        for (int k = 0; k < 3; ++k) start[k] = floor_u32(static_cast<float>(pos[k] - pos[k] % size));
+      } else {
+        size = 2;
+        const V3f rel = to_f(pos) - bounds.min;
+        const V3f s2 = ((rel * 2.0f) * static_cast<float>(dim)) / bounds.size;
+        start[0] = floor_u32(s2.x);
+        start[1] = floor_u32(s2.y);
+        start[2] = floor_u32(s2.z);
+        for (int k = 0; k < 3; ++k) start[k] -= start[k] % 2u;
+      }
+      break;
+  }
+  uint32_t color = 0;
+  bool have = false;
+  if (n.kind == NodeKind::Leaf || bottom) {
+    have = resample(sampler, start, size,
+                    [&](uint32_t x, uint32_t y, uint32_t z, uint32_t* c) {
+                      const uint32_t q[3] = {x, y, z};
+                      return albedo_of(get_internal(key, bounds, q), c);
+                    },
+                    &color);
+  } else {
+    const float fdim = static_cast<float>(dim);
+    const NodeChildren& kids_of = node_children_[key];
+    const V3f sf{static_cast<float>(start[0]), static_cast<float>(start[1]),
+                 static_cast<float>(start[2])};
+    const uint32_t first = hash_region(sf, fdim);
+    if (first < 8u && kids_of.child(first) != EMPTY_MARKER) {
+      have = resample(sampler, start, size,
+                      [&](uint32_t x, uint32_t y, uint32_t z, uint32_t* c) {
+                        const V3f pf{static_cast<float>(x), static_cast<float>(y),
+                                     static_cast<float>(z)};
+                        const uint32_t oct = hash_region(pf, fdim);
+                        if (oct >= 8u) return false;
+                        const uint32_t child = kids_of.child(oct);
+                        if (child == EMPTY_MARKER) return false;
+                        const V3f off = octant_offset(oct) * fdim;
+                        const uint32_t q[3] = {x - round_u32(off.x), y - round_u32(off.y),
+                                               z - round_u32(off.z)};
+                        const Brick& m = node_mip(child);
+                        if (m.kind == BrickKind::Empty) return false;
+                        const uint32_t pix =
+                            m.kind == BrickKind::Solid
+                                ? m.solid
+                                : m.parted[flat_projection(q[0], q[1], q[2], dim)];
+                        return albedo_of(pix_get_ref(pix), c);
+                      },
+                      &color);
+    }
+  }
+  if (!have) return;
+  // colour matching against the palette (mipmap.rs:527-560)
+  uint32_t entry;
+  auto thr = mip_strategy_.color_thresholds.find(level);
+  bool similar = false;
+  if (thr != mip_strategy_.color_thresholds.end()) {
+    const float limit = thr->second * 255.0f;
+    for (uint32_t i = 0; i < color_palette_.size(); ++i)
+      if (color_distance(color, color_palette_[i]) < limit) {
+        entry = (i & 0xFFFFu) | (NONE16 << 16);  // NodeContent::pix_visual(i as u16)
+        similar = true;
+        break;
+      }
+  }
+  if (!similar) entry = add_to_palette(Entry{EntryKind::Visual, color, 0});
+  uint64_t mi[3];
+  matrix_index_for(bounds, pos, dim, mi);
+  const uint64_t f = flat_projection(mi[0], mi[1], mi[2], dim);
+  Brick& m = mip_mut(key);
+  if (m.kind == BrickKind::Empty) {
+    m = Brick::make_parted(std::vector<uint32_t>(static_cast<size_t>(dim) * dim * dim, EMPTY_MARKER));
+  } else if (m.kind == BrickKind::Solid) {
+    m = Brick::make_parted(std::vector<uint32_t>(static_cast<size_t>(dim) * dim * dim, m.solid));
+  }
+  m.parted[f] = entry;
+}
+
+Entry Octree::sample_root_mip(uint32_t octant, uint32_t x, uint32_t y, uint32_t z) const {
+  const uint32_t key =
+      octant == OOB_OCTANT ? ROOT_NODE_KEY : node_children_[ROOT_NODE_KEY].child(octant);
+  if (!nodes_.key_is_valid(key)) return Entry{};
+  const Brick& m = node_mip(key);
+  if (m.kind == BrickKind::Empty) return Entry{};
+  if (m.kind == BrickKind::Solid) return pix_get_ref(m.solid);
+  return pix_get_ref(m.parted[flat_projection(x, y, z, brick_dim_)]);
 }
 
 }  // namespace svo

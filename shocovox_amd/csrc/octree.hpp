@@ -14,6 +14,7 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -170,6 +171,37 @@ class NodePool {
   std::vector<uint32_t> dirty_;
 };
 
+// MIPResamplingMethods (types.rs:104-140): the threshold of the Posterize
+// forms is kept clamped to [0, 1].
+enum class MipMethod : uint8_t {
+  BoxFilter = 0,
+  PointFilter = 1,
+  PointFilterBD = 2,
+  Posterize = 3,
+  PosterizeBD = 4
+};
+struct MipMethodSpec {
+  MipMethod kind = MipMethod::BoxFilter;
+  float thr = 0.0f;
+  bool operator==(const MipMethodSpec& o) const { return kind == o.kind && thr == o.thr; }
+};
+
+// MIPMapStrategy (types.rs:141-160) with its default (mipmap.rs:591-604):
+// disabled; level 1 PointFilter, levels 2-4 BoxFilter (others BoxFilter too);
+// colour-matching thresholds 0.1, 0.05 and 0.02 at levels 2, 3 and 4.
+struct MipStrategy {
+  bool enabled = false;
+  std::map<uint32_t, MipMethodSpec> methods{{1, {MipMethod::PointFilter, 0.0f}},
+                                            {2, {MipMethod::BoxFilter, 0.0f}},
+                                            {3, {MipMethod::BoxFilter, 0.0f}},
+                                            {4, {MipMethod::BoxFilter, 0.0f}}};
+  std::map<uint32_t, float> color_thresholds{{2, 0.1f}, {3, 0.05f}, {4, 0.02f}};
+  MipMethodSpec method_at(uint32_t level) const {
+    auto it = methods.find(level);
+    return it == methods.end() ? MipMethodSpec{} : it->second;
+  }
+};
+
 class Octree {
  public:
   static constexpr uint32_t ROOT_NODE_KEY = 0;
@@ -190,6 +222,23 @@ class Octree {
   Status clear(uint32_t x, uint32_t y, uint32_t z) { return clear_at_lod(x, y, z, 1); }
   Status clear_at_lod(uint32_t x, uint32_t y, uint32_t z, uint32_t size);
   Entry get(uint32_t x, uint32_t y, uint32_t z) const;
+
+  // MIP maps (mipmap.rs). The strategy setters follow StrategyUpdater
+  // (mipmap.rs:706-797); switch_albedo_mip_maps recalculates every MIP when
+  // MIPs are turned on for a non-empty tree (:865-880).
+  const MipStrategy& mip_strategy() const { return mip_strategy_; }
+  void set_mip_method(uint32_t level, MipMethodSpec m);
+  void set_mip_color_threshold(uint32_t level, float thr);
+  void reset_mip_strategy();
+  void switch_albedo_mip_maps(bool enabled);
+  void recalculate_mips();
+  // node_mips[key]; keys never written hold an Empty brick
+  const Brick& node_mip(uint32_t key) const;
+  size_t node_mip_count() const { return node_mips_.size(); }
+  // sample_root_mip (mipmap.rs:905-937): octant 8 samples the root's MIP
+  Entry sample_root_mip(uint32_t octant, uint32_t x, uint32_t y, uint32_t z) const;
+  // Installs deserialized MIPs and strategy.
+  void assign_mips(std::vector<Brick> mips, MipStrategy strategy);
 
   uint32_t size() const { return octree_size_; }
   uint32_t brick_dim() const { return brick_dim_; }
@@ -231,6 +280,10 @@ class Octree {
                                Cube brick_bounds, uint32_t dim, const uint32_t pos[3],
                                uint32_t size, uint32_t data);
   void subdivide_leaf_to_nodes(uint32_t node_key, uint32_t target_octant);
+  Entry get_internal(uint32_t key, Cube bounds, const uint32_t pos[3]) const;
+  void update_mip(uint32_t key, Cube bounds, const uint32_t pos[3]);
+  void recalculate_mip(uint32_t key, Cube bounds);
+  Brick& mip_mut(uint32_t key);
   void deallocate_children_of(uint32_t node);
   Brick try_brick_from_node(uint32_t key) const;
   uint32_t push_node(NodeContent c);
@@ -262,6 +315,8 @@ class Octree {
   std::unordered_map<uint32_t, uint32_t> color_index_;
   std::unordered_map<uint32_t, uint32_t> data_index_;
   uint64_t edit_epoch_ = 0;
+  MipStrategy mip_strategy_;
+  std::vector<Brick> node_mips_{Brick{}};
 };
 
 // Helpers of src/spatial/math/mod.rs used by the builder.

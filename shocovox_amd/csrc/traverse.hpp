@@ -40,6 +40,7 @@ struct TreeView {
   float inv_dim;  // pow2_rcp(dim): exact since the brick dimension is a power of two
   uint32_t node_count;
   uint32_t max_steps;  // iteration bound of one ray (MAX_TRAVERSAL_STEPS unless lowered)
+  uint32_t mips_enabled;  // MIPMapStrategy::enabled: LOD queries probe node MIPs
 };
 
 struct CastResult {
@@ -129,17 +130,23 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   return false;
 }
 
-// get_by_ray_at_lod(ray, f32::MAX) with MIPs disabled. `lut` is the
-// RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the caller staged it.
-// `top` optionally holds a copy of the first `ntop` node records (the top
-// levels, as records are numbered breadth-first), e.g. staged in LDS; those
-// records are read from it instead of t.nodes.
-template <class LutT>
-SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t,
-                                                                const LutT* __restrict__ lut,
-                                                                V3f o, V3f d,
-                                                                const NodeRec* top = nullptr,
-                                                                uint32_t ntop = 0u) {
+// get_by_ray_at_lod (raytracing_on_cpu.rs:325-565). Without LOD it is
+// get_by_ray (viewing distance f32::MAX, under which the MIP test never
+// passes). `lut` is the RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the
+// caller staged it. `top` optionally holds a copy of the first `ntop` node
+// records (the top levels, as records are numbered breadth-first), e.g.
+// staged in LDS; those records are read from it instead of t.nodes.
+//
+// With LOD and the tree's MIPs enabled, every stack-loop iteration first
+// tests the node's level against the distance travelled (:368-386): the
+// level starts at log2(size / dim), falls by one per PUSH, rises by one per
+// POP and is not reset by a restart from the root; when the level is below
+// |origin - round(p / (2 level)) * 2 level| / viewing_distance, the node's
+// MIP brick is probed over the node's bounds like a uniform leaf's brick.
+template <bool LOD, class LutT>
+SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
+    const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
+    uint32_t ntop, float viewing_distance) {
   CastResult r;
   r.hit = 0;
   r.pix = EMPTY_MARKER;
@@ -172,6 +179,9 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
     }
   }
   uint32_t steps = 0;
+  // log2(size / dim) of two powers of two, exactly
+  float mip_level =
+      LOD ? static_cast<float>(__builtin_ctz(static_cast<uint32_t>(t.size) / t.dim)) : 0.0f;
   while (target != OOB_OCTANT) {
     bounds = root;
     head = (head + 1u) & 3u;
@@ -188,6 +198,16 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
       const uint32_t meta = n.meta;
       const uint32_t kind = node_kind(meta);
       bool backtrack = kind == KIND_UNIFORM_LEAF;
+      if (LOD && t.mips_enabled) {
+        const float grid = mip_level * 2.0f;
+        const V3f snapped{roundf(p.x / grid) * grid, roundf(p.y / grid) * grid,
+                          roundf(p.z / grid) * grid};
+        if (mip_level < length(o - snapped) / viewing_distance &&
+            probe_brick(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r)) {
+          r.hit = 1;
+          return r;
+        }
+      }
       if (target != OOB_OCTANT) {
         if (kind == KIND_UNIFORM_LEAF) {
           if (probe_brick(t, brick_form(meta, 0), n.child[0], bounds, d, sig, scale, &p, &r)) {
@@ -213,6 +233,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
         // POP (:445-475)
         head = (head + 3u) & 3u;
         count -= 1u;
+        if (LOD) mip_level += 1.0f;
         if (count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);
           const float s2 = bounds.size * 2.0f;
@@ -240,6 +261,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
         head = (head + 1u) & 3u;
         count = count < 4u ? count + 1u : 4u;
         stk[head] = tck;
+        if (LOD) mip_level -= 1.0f;
       } else {
         // ADVANCE (:497-544)
         for (;;) {
@@ -271,6 +293,21 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
                  : OOB_OCTANT;
   }
   return r;
+}
+
+template <class LutT>
+SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t,
+                                                                const LutT* __restrict__ lut,
+                                                                V3f o, V3f d,
+                                                                const NodeRec* top = nullptr,
+                                                                uint32_t ntop = 0u) {
+  return cast_ray_impl<false>(t, lut, o, d, top, ntop, 0.0f);
+}
+
+template <class LutT>
+SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_lod(
+    const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, float viewing_distance) {
+  return cast_ray_impl<true>(t, lut, o, d, nullptr, 0u, viewing_distance);
 }
 
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t, V3f o,

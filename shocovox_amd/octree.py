@@ -96,6 +96,90 @@ def _check(status: int) -> None:
         raise OctreeError(status, msg.decode() if msg else "")
 
 
+@dataclass(frozen=True)
+class MIPResamplingMethods:
+    """MIPResamplingMethods (octree/types.rs:104-140): BoxFilter, PointFilter,
+    PointFilterBD, Posterize(thr), PosterizeBD(thr)."""
+    kind: int
+    thr: float = 0.0
+
+    @staticmethod
+    def Posterize(thr: float) -> "MIPResamplingMethods":  # noqa: N802 (reference name)
+        return MIPResamplingMethods(3, float(thr))
+
+    @staticmethod
+    def PosterizeBD(thr: float) -> "MIPResamplingMethods":  # noqa: N802
+        return MIPResamplingMethods(4, float(thr))
+
+
+MIPResamplingMethods.BoxFilter = MIPResamplingMethods(0)
+MIPResamplingMethods.PointFilter = MIPResamplingMethods(1)
+MIPResamplingMethods.PointFilterBD = MIPResamplingMethods(2)
+
+
+class StrategyUpdater:
+    """The chainable MIP strategy editor Octree::albedo_mip_map_resampling_strategy
+    returns (octree/mod.rs:379-381, mipmap.rs:706-937)."""
+
+    def __init__(self, tree: "Octree"):
+        self._t = tree
+
+    def _h(self):
+        return self._t.handle
+
+    def switch_albedo_mip_maps(self, enabled: bool) -> "StrategyUpdater":
+        _check(N.lib().svo_octree_switch_mip_maps(self._h(), int(bool(enabled))))
+        return self
+
+    def is_enabled(self) -> bool:
+        return bool(N.lib().svo_octree_mip_maps_enabled(self._h()))
+
+    def set_method_at(self, mip_level: int, method: MIPResamplingMethods) -> "StrategyUpdater":
+        _check(N.lib().svo_octree_set_mip_method(self._h(), int(mip_level), method.kind,
+                                                 float(method.thr)))
+        return self
+
+    def set_method(self, levels) -> "StrategyUpdater":
+        for level, method in (levels.items() if isinstance(levels, dict) else levels):
+            self.set_method_at(level, method)
+        return self
+
+    def get_method_at(self, mip_level: int) -> MIPResamplingMethods:
+        m, t = C.c_uint32(), C.c_float()
+        _check(N.lib().svo_octree_get_mip_method(self._h(), int(mip_level), C.byref(m),
+                                                 C.byref(t)))
+        return MIPResamplingMethods(int(m.value), float(t.value) if m.value >= 3 else 0.0)
+
+    def set_color_similarity_thr_at(self, mip_level: int, thr: float) -> "StrategyUpdater":
+        _check(N.lib().svo_octree_set_mip_color_threshold(self._h(), int(mip_level), float(thr)))
+        return self
+
+    def set_color_similarity_thr(self, levels) -> "StrategyUpdater":
+        for level, thr in (levels.items() if isinstance(levels, dict) else levels):
+            self.set_color_similarity_thr_at(level, thr)
+        return self
+
+    def get_new_color_similarity_at(self, mip_level: int) -> float:
+        t = C.c_float()
+        _check(N.lib().svo_octree_get_mip_color_threshold(self._h(), int(mip_level), C.byref(t)))
+        return float(t.value)
+
+    def reset(self) -> "StrategyUpdater":
+        _check(N.lib().svo_octree_reset_mip_strategy(self._h()))
+        return self
+
+    def recalculate_mips(self) -> "StrategyUpdater":
+        _check(N.lib().svo_octree_recalculate_mips(self._h()))
+        return self
+
+    def sample_root_mip(self, octant: int, position) -> "OctreeEntry":
+        e = N.Entry()
+        _check(N.lib().svo_octree_sample_root_mip(self._h(), int(octant), int(position[0]),
+                                                  int(position[1]), int(position[2]),
+                                                  C.byref(e)))
+        return OctreeEntry.from_c(e)
+
+
 class Octree:
     """Host octree, Octree<u32> of the reference."""
 
@@ -207,6 +291,10 @@ class Octree:
         _check(N.lib().svo_octree_get(self._h, pos[0], pos[1], pos[2], C.byref(e)))
         return OctreeEntry.from_c(e)
 
+    def albedo_mip_map_resampling_strategy(self) -> StrategyUpdater:
+        """Octree::albedo_mip_map_resampling_strategy (octree/mod.rs:379-381)."""
+        return StrategyUpdater(self)
+
     def insert_many(self, xyz: np.ndarray, entries: np.ndarray) -> None:
         """insert() for each row of xyz (n,3 u32) with entries (n,3 u32: kind, albedo, data)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.uint32)
@@ -241,15 +329,20 @@ class Octree:
             "voxels": np.zeros(max(1, s.parted_count * dim ** 3), np.uint32),
             "colors": np.zeros(max(1, s.color_count), np.uint32),
             "data": np.zeros(max(1, s.data_count), np.uint32),
+            "mip_kind": np.zeros(max(1, n), np.uint8), "mip_value": np.zeros(max(1, n), np.uint32),
+            "mip_voxels": np.zeros(max(1, s.mip_parted_count * dim ** 3), np.uint32),
         }
         d = N.ModelDump(*(out[k].ctypes.data for k in (
             "reserved", "content_kind", "internal_bits", "children_kind", "children",
-            "children_bitmap", "brick_kind", "brick_value", "voxels", "colors", "data")))
+            "children_bitmap", "brick_kind", "brick_value", "voxels", "colors", "data",
+            "mip_kind", "mip_value", "mip_voxels")))
         _check(lib.svo_octree_export(self._h, C.byref(s), C.byref(d)))
         out["pool_len"] = n
         out["parted_count"] = s.parted_count
         out["color_count"] = s.color_count
         out["data_count"] = s.data_count
+        out["mip_parted_count"] = s.mip_parted_count
+        out["mips_enabled"] = s.mips_enabled
         return out
 
     def flatten(self) -> "FlatTree":
@@ -258,8 +351,8 @@ class Octree:
         return FlatTree(h.value, self)
 
 
-NODE_DTYPE = np.dtype([("occ", "<u8"), ("meta", "<u4"), ("reserved0", "<u4"),
-                       ("child", "<u4", (8,)), ("reserved1", "<u4", (4,))])
+NODE_DTYPE = np.dtype([("occ", "<u8"), ("meta", "<u4"), ("mip", "<u4"),
+                       ("child", "<u4", (8,)), ("mip_word", "<u4"), ("reserved", "<u4", (3,))])
 
 
 class FlatTree:
@@ -298,6 +391,7 @@ class FlatTree:
             "brick_pix": view(d.brick_pix, d.brick_count * n3, np.uint32),
             "brick_occ": view(d.brick_occ, d.brick_count * words, np.uint32),
             "colors": view(d.colors, d.color_count, np.uint32),
+            "mips_enabled": int(d.mips_enabled),
         }
 
     def sync(self, octree: Octree) -> None:
@@ -442,6 +536,29 @@ class DeviceTree:
                stream=None) -> dict:
         bufs = alloc_hits(w * h, device, fields)
         self.render_into(vp, w, h, bufs, stream)
+        return bufs
+
+    def cast_at_lod(self, rays, viewing_distance: float, fields=HIT_FIELDS,
+                    stream=None) -> dict:
+        """get_by_ray_at_lod (raytracing_on_cpu.rs:325-565) for every row of
+        `rays` (n x 6 f32 device tensor)."""
+        torch = _torch()
+        rays = rays.to(dtype=torch.float32).contiguous()
+        n = rays.shape[0]
+        bufs = alloc_hits(n, rays.device, fields)
+        hs = _hits_struct(bufs)
+        _check(N.lib().svo_cast_at_lod(self._h, rays.data_ptr(), n, float(viewing_distance),
+                                       C.byref(hs), _stream_ptr(stream)))
+        return bufs
+
+    def render_at_lod(self, vp: Viewport, w: int, h: int, viewing_distance: float,
+                      fields=HIT_FIELDS, device="cuda", stream=None) -> dict:
+        """get_by_ray_at_lod for the primary rays of a viewport."""
+        bufs = alloc_hits(w * h, device, fields)
+        hs = _hits_struct(bufs)
+        v = vp.to_c()
+        _check(N.lib().svo_render_at_lod(self._h, C.byref(v), w, h, float(viewing_distance),
+                                         C.byref(hs), _stream_ptr(stream)))
         return bufs
 
 

@@ -19,7 +19,8 @@ class _Tree(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("brick_pix", C.c_void_p), ("brick_occ", C.c_void_p),
                 ("colors", C.c_void_p), ("ray_lut", C.c_void_p), ("size", C.c_float),
                 ("dim", C.c_uint32), ("occ_words", C.c_uint32), ("color_count", C.c_uint32),
-                ("max_steps", C.c_uint32)]
+                ("max_steps", C.c_uint32), ("mips_enabled", C.c_uint32),
+                ("viewing_distance", C.c_float)]
 
 
 _lib = None
@@ -44,18 +45,21 @@ def ray_lut() -> np.ndarray:
 
 
 def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False,
-         max_steps: int = 0) -> np.ndarray:
-    """The product traversal on the host: cast_ray, or with resumable=True
-    its ray_begin / ray_step form."""
+         max_steps: int = 0, viewing_distance: float | None = None) -> np.ndarray:
+    """The product traversal on the host: cast_ray, with resumable=True its
+    ray_begin / ray_step form, with a viewing distance cast_ray_lod."""
     a = flat_arrays
     keep = [np.ascontiguousarray(a["nodes"]), np.ascontiguousarray(a["brick_pix"]),
             np.ascontiguousarray(a["brick_occ"]), np.ascontiguousarray(a["colors"]), ray_lut()]
     dim = a["brick_dim"]
     t = _Tree(*(k.ctypes.data for k in keep), float(a["octree_size"]), dim,
-              (dim ** 3 + 31) // 32, len(a["colors"]), max_steps)
+              (dim ** 3 + 31) // 32, len(a["colors"]), max_steps,
+              int(a.get("mips_enabled", 0)),
+              float(viewing_distance) if viewing_distance is not None else 0.0)
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
     out = np.zeros(len(rays), HIT_DTYPE)
-    lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, int(resumable))
+    mode = 2 if viewing_distance is not None else int(resumable)
+    lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, mode)
     return out
 
 

@@ -19,6 +19,8 @@ struct hs_tree {
   uint32_t occ_words;
   uint32_t color_count;
   uint32_t max_steps;  // 0: the default bound
+  uint32_t mips_enabled;
+  float viewing_distance;  // mode 2
 };
 
 struct hs_hit {
@@ -30,17 +32,20 @@ struct hs_hit {
 };
 
 // mode 0: cast_ray; mode 1: the resumable form (ray_begin, then ray_step
-// until the ray finishes), as the refill kernels run it.
+// until the ray finishes), as the refill kernels run it; mode 2: cast_ray_lod
+// with t->viewing_distance.
 void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode) {
   svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut, t->size, t->dim,
                   t->occ_words, t->color_count, svo::pow2_rcp(static_cast<float>(t->dim)), 0u,
-                  t->max_steps ? t->max_steps : svo::MAX_TRAVERSAL_STEPS};
+                  t->max_steps ? t->max_steps : svo::MAX_TRAVERSAL_STEPS, t->mips_enabled};
   for (uint64_t i = 0; i < n; ++i) {
     const float* r = rays + 6 * i;
     const svo::V3f o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
     svo::CastResult c;
     if (mode == 0) {
       c = svo::cast_ray(v, o, d);
+    } else if (mode == 2) {
+      c = svo::cast_ray_lod(v, v.ray_lut, o, d, t->viewing_distance);
     } else {
       svo::RayState st;
       if (svo::ray_begin(v, o, d, st)) {

@@ -461,3 +461,34 @@ def test_vox_scene_renders_like_the_oracle(gpu_device, name, dim):
     got = gpu_to_numpy(d.cast(torch.from_numpy(inc).to(gpu_device)))
     torch.cuda.synchronize()
     assert_parity(_oracle(tree).cast(inc), got, f"vox {name} incoherent")
+
+
+@pytest.mark.parametrize("vd", [0.5, 8.0, 1e6])
+def test_lod_traversal_on_device(gpu_device, vd):
+    """get_by_ray_at_lod on the device (svo_cast_at_lod, svo_render_at_lod)
+    with MIPs enabled equals the oracle; after in-place edits (MIP bricks
+    patched with the records) it still does."""
+    import torch
+    tree = Octree.scene(2, 256, 8, 12)
+    tree.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    d = DeviceTree(tree)
+    vp = W.look_at_viewport((330.0, 240.0, 300.0), (128.0, 128.0, 128.0), 320, 200)
+    prim = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 320, 200)
+    inc = W.incoherent_rays(256, 100_000, seed=31)
+    rng = np.random.default_rng(7)
+    for rnd in range(2):
+        _say(f"lod vd={vd} round {rnd} render")
+        got = gpu_to_numpy(d.render_at_lod(vp, 320, 200, vd))
+        torch.cuda.synchronize()
+        o = _oracle(tree)
+        assert_parity(o.cast(prim, viewing_distance=vd), got, f"lod render vd={vd} r{rnd}")
+        _say(f"lod vd={vd} round {rnd} cast")
+        got = gpu_to_numpy(d.cast_at_lod(torch.from_numpy(inc).to(gpu_device), vd))
+        torch.cuda.synchronize()
+        assert_parity(o.cast(inc, viewing_distance=vd), got, f"lod cast vd={vd} r{rnd}")
+        xyz = rng.integers(0, 256, size=(500, 3)).astype(np.uint32)
+        ents = np.zeros((500, 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = 0x8844CCFF
+        tree.insert_many(xyz, ents)
+        d.update(tree)

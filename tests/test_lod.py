@@ -1,0 +1,141 @@
+"""get_by_ray_at_lod with MIP maps (raytracing_on_cpu.rs:325-565 with the
+MIP branch of :368-386): the oracle's restatement, pinned by hand-derived
+cases and invariants, and the product traversal (traverse.hpp compiled for
+the host) checked against it bit for bit."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import hostsim
+import oracle
+from parity import assert_parity as _assert_parity
+from shocovox_amd import MIPResamplingMethods as M, Octree, OctreeEntry, workloads as W
+
+F32_MAX = float(np.finfo(np.float32).max)
+RED = 0xFF0000FF
+
+
+def assert_parity(expected: np.ndarray, got, label: str = ""):
+    """Bit-for-bit equality of two result arrays of the hit layout."""
+    if isinstance(got, np.ndarray):
+        got = {k: got[k] for k in got.dtype.names}
+    return _assert_parity(expected, got, label)
+
+
+def _oracle(t: Octree) -> oracle.OracleTree:
+    return oracle.OracleTree(t.export_model())
+
+
+def _single_voxel_tree() -> Octree:
+    t = Octree.new(4, 2)
+    t.set_auto_simplify(False)
+    t.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True).set_method_at(
+        1, M.BoxFilter)
+    t.insert((0, 0, 0), OctreeEntry.visual(RED))
+    return t
+
+
+def test_far_ray_hits_the_root_mip_cell():
+    """Tree 4^3, brick dim 2, one red voxel at the origin: the root's MIP
+    cell (0,0,0) covers [0,2)^3. A ray along +x at y = z = 1.5 from x = -50
+    misses the voxel, but enters the root at (0, 1.5, 1.5), whose level-1
+    grid point is (0, 2, 2): 50.005 / viewing distance 1 > level 1, so the
+    root MIP is probed and the cell answers at x = 0 with normal -x. At
+    viewing distance 1000 the test fails (0.05 < 1) and the ray misses."""
+    t = _single_voxel_tree()
+    assert t.albedo_mip_map_resampling_strategy().sample_root_mip(8, (0, 0, 0)).albedo == RED
+    ray = np.array([[-50.0, 1.5, 1.5, 1.0, 0.0, 0.0]], np.float32)
+    o = _oracle(t)
+    far = o.cast(ray, viewing_distance=1.0)[0]
+    assert far["hit"] == 1 and far["albedo"] == RED
+    assert tuple(far["impact"]) == (0.0, 1.5, 1.5) and tuple(far["normal"]) == (-1.0, 0.0, 0.0)
+    assert o.cast(ray, viewing_distance=1000.0)[0]["hit"] == 0
+    assert o.cast(ray)[0]["hit"] == 0
+    # the full-resolution voxel is still what a ray through it hits
+    thru = np.array([[-50.0, 0.5, 0.5, 1.0, 0.0, 0.0]], np.float32)
+    assert o.cast(thru)[0]["hit"] == 1
+    f = t.flatten().arrays()
+    for vd in (1.0, 1000.0):
+        assert_parity(o.cast(ray, viewing_distance=vd),
+                      hostsim.cast(f, ray, viewing_distance=vd), f"kat vd={vd}")
+
+
+def test_mips_off_or_infinite_distance_is_get_by_ray():
+    """With MIPs disabled the viewing distance changes nothing; with MIPs on,
+    a viewing distance of f32::MAX never passes the level test."""
+    t = Octree.scene(2, 128, 8, 3)
+    rays = W.incoherent_rays(128, 3000, seed=11)
+    base = _oracle(t).cast(rays)
+    for vd in (0.5, 10.0, 1e4):
+        assert_parity(base, _oracle(t).cast(rays, viewing_distance=vd), f"mips off vd={vd}")
+    t.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    assert_parity(base, _oracle(t).cast(rays, viewing_distance=F32_MAX), "mips on, f32::MAX")
+
+
+def _lod_scene(kind: int, size: int, dim: int, seed: int) -> Octree:
+    t = Octree.scene(kind, size, dim, seed)
+    t.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    return t
+
+
+@pytest.mark.parametrize("kind,size,dim,seed", [(1, 64, 4, 1), (2, 128, 8, 2), (2, 128, 16, 4)])
+def test_lod_hits_come_from_mip_bricks(kind, size, dim, seed):
+    """At a small viewing distance every ray that now differs from get_by_ray
+    ends on a palette value held by some node's MIP brick."""
+    t = _lod_scene(kind, size, dim, seed)
+    rays = W.incoherent_rays(size, 4000, seed=seed + 20)
+    o = _oracle(t)
+    full = o.cast(rays)
+    lod = o.cast(rays, viewing_distance=2.0)
+    m = t.export_model()
+    n3 = dim ** 3
+    mip_pix = set(m["mip_value"][m["mip_kind"] == 1].tolist())
+    mip_pix |= set(m["mip_voxels"][:m["mip_parted_count"] * n3].tolist())
+    diff = (full["hit"] != lod["hit"]) | (full["pix"] != lod["pix"]) | \
+        np.any(full["impact"] != lod["impact"], axis=1)
+    assert diff.any()
+    changed_hits = diff & (lod["hit"] == 1)
+    assert changed_hits.any()
+    assert set(lod["pix"][changed_hits].tolist()) <= mip_pix
+
+
+@pytest.mark.parametrize("kind,size,dim,seed", [(1, 64, 4, 1), (2, 128, 8, 2), (2, 128, 16, 4),
+                                                (2, 256, 16, 7)])
+@pytest.mark.parametrize("vd", [0.25, 4.0, 64.0, 1e6])
+def test_product_lod_traversal_equals_the_oracle(kind, size, dim, seed, vd):
+    """The product's cast_ray_lod (host build of traverse.hpp) on the
+    flattened layout equals the oracle on the enum model, for incoherent and
+    primary rays."""
+    t = _lod_scene(kind, size, dim, seed)
+    f = t.flatten().arrays()
+    assert f["mips_enabled"] == 1
+    rays = W.incoherent_rays(size, 3000, seed=seed + 40)
+    vp = W.look_at_viewport((1.4 * size, 0.9 * size, 1.2 * size), (size / 2,) * 3, 96, 64)
+    prim = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 96, 64)
+    o = _oracle(t)
+    for name, r in (("incoherent", rays), ("primary", prim)):
+        assert_parity(o.cast(r, viewing_distance=vd), hostsim.cast(f, r, viewing_distance=vd),
+                      f"{name} vd={vd}")
+
+
+def test_edited_mips_reach_the_mirror():
+    """Edits update MIPs incrementally; the layout synced edit by edit
+    (LayoutMirror, MIP brick slots included) traverses at LOD exactly as the
+    oracle does on the edited tree."""
+    t = _lod_scene(2, 128, 8, 9)
+    flat = t.flatten()
+    rng = np.random.default_rng(3)
+    rays = W.incoherent_rays(128, 2000, seed=9)
+    for rnd in range(3):
+        xyz = rng.integers(0, 128, size=(300, 3)).astype(np.uint32)
+        ents = np.zeros((300, 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = 0x22AA44FF + (rnd << 24)
+        if rnd == 1:
+            t.clear_many(xyz)
+        else:
+            t.insert_many(xyz, ents)
+        flat.sync(t)
+        got = hostsim.cast(flat.arrays(), rays, viewing_distance=3.0)
+        assert_parity(_oracle(t).cast(rays, viewing_distance=3.0), got, f"round {rnd}")
