@@ -85,7 +85,6 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   if (form == FORM_SOLID) {
     r->pix = word;
     r->impact = *p;
-    r->normal = cube_impact_normal(bb, *p);
     r->cell = bb;
     return true;
   }
@@ -104,10 +103,9 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
                        unit,
           unit};
   const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
-  V3f step{0.0f, 0.0f, 0.0f};
   for (int guard = 0; guard < 4 * dim + 16; ++guard) {
     if (ix < 0 || ix >= dim || iy < 0 || iy >= dim || iz < 0 || iz >= dim) return false;
-    flat += step_int(step.x) * 1 + step_int(step.y) * dim + step_int(step.z) * (dim * dim);
+    flat = ix + iy * dim + iz * dim * dim;
     if ((occ[static_cast<uint32_t>(flat) >> 5] >> (static_cast<uint32_t>(flat) & 31u)) & 1u) {
       const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
       r->pix = t.brick_pix[static_cast<uint64_t>(word) * n3 + static_cast<uint32_t>(flat)];
@@ -116,11 +114,10 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
                          bb.size;
       const Cube hb{bb.min + div3_by(scaled, fdim, t.inv_dim), div_by(bb.size, fdim, t.inv_dim)};
       r->impact = *p;
-      r->normal = cube_impact_normal(hb, *p);
       r->cell = hb;
       return true;
     }
-    step = dda_step(d, sig, p, cb, scale);
+    const V3f step = dda_step(d, sig, p, cb, scale);
     cb.min = cb.min + step * unit;
     ix += step_int(step.x);
     iy += step_int(step.y);
@@ -203,25 +200,16 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
         const V3f snapped{roundf(p.x / grid) * grid, roundf(p.y / grid) * grid,
                           roundf(p.z / grid) * grid};
         if (mip_level < length(o - snapped) / viewing_distance &&
-            probe_brick(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r)) {
-          r.hit = 1;
-          return r;
-        }
+            probe_brick(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r))
+          goto hit_found;
       }
-      if (target != OOB_OCTANT) {
-        if (kind == KIND_UNIFORM_LEAF) {
-          if (probe_brick(t, brick_form(meta, 0), n.child[0], bounds, d, sig, scale, &p, &r)) {
-            r.hit = 1;
-            return r;
-          }
-          backtrack = true;
-        } else if (kind == KIND_LEAF) {
-          if (probe_brick(t, brick_form(meta, target), n.child[target],
-                          child_bounds_for(bounds, target), d, sig, scale, &p, &r)) {
-            r.hit = 1;
-            return r;
-          }
-        }
+      if (target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
+        const bool uniform = kind == KIND_UNIFORM_LEAF;
+        const uint32_t slot = uniform ? 0u : target;
+        if (probe_brick(t, brick_form(meta, slot), n.child[slot],
+                        uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale, &p,
+                        &r))
+          goto hit_found;
       }
       const float rcp_size = pow2_rcp(bounds.size);
       const V3f rel = (p - bounds.min) * 4.0f;
@@ -292,6 +280,11 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
                  ? hash_region(p, t.size / 2.0f)
                  : OOB_OCTANT;
   }
+  return r;
+hit_found:
+  // the normal is derived once, after the loops, from the hit cell and point
+  r.hit = 1;
+  r.normal = cube_impact_normal(r.cell, r.impact);
   return r;
 }
 
@@ -403,6 +396,7 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
                     s.scale, &s.p, &hr)) {
       r = hr;
       r.hit = 1;
+      r.normal = cube_impact_normal(hr.cell, hr.impact);
       r.flags = s.flags | hr.flags;
       return true;
     }
