@@ -226,6 +226,9 @@ const RenderVariant kRenderVariants[] = {
     {"w5-glut-8x8-n73-xcd", render_kernel<5, false, 8, 8, 73, true>, 8, 8},
     {"w5-slut-8x8-n73", render_kernel<5, true, 8, 8, 73>, 8, 8},
     {"w1-slut-8x8-n73", render_kernel<1, true, 8, 8, 73>, 8, 8},
+    {"w6-glut-8x8-n73", render_kernel<6, false, 8, 8, 73>, 8, 8},
+    {"w6-slut-8x8-n73", render_kernel<6, true, 8, 8, 73>, 8, 8},
+    {"w6-glut-8x8-xcd", render_kernel<6, false, 8, 8, 0, true>, 8, 8},
     // lane refill measured 3.5x slower than one tile per wave on coherent
     // 1080p primary rays (lanes drift apart and lose the shared node lines);
     // kept as a variant for divergent workloads
