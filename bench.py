@@ -203,8 +203,10 @@ def batched_bench(args, render_variant: str) -> dict:
 def served_bench(args, capacity_mrays: float) -> dict:
     """Served mode: a pool of resident trees admitted by free HBM, then the
     same seeded Poisson plan (Zipf(1.1) over trees and resolutions, 5% sparse
-    edits of 1-1000 voxels) served by continuous batching and by per-request
-    plain launches, at fixed fractions of the measured render capacity."""
+    edits of 1-1000 voxels) served by continuous batching (mode "batched",
+    which runs a launch holding one whole request as a plain render, and
+    "batched-kernel", which never does) and by per-request plain launches, at
+    fixed fractions of the measured render capacity."""
     import numpy as np
 
     from shocovox_amd import Octree, batch_variants, current_batch_variant, serve
@@ -220,7 +222,10 @@ def served_bench(args, capacity_mrays: float) -> dict:
     cfg0 = serve.ServeConfig(qps=1.0, duration_s=1.0, seed=args.seed,
                              max_inflight=args.served_inflight,
                              max_batch_rays=args.served_batch_rays)
-    # device memory kept free for the loop's output pools (either mode) and a margin
+    modes = ("batched", "plain", "batched-kernel")
+    # device memory kept free for the loop's output pools and a margin; every
+    # mode gets its own copy of the pool, so each starts from the same trees
+    # and sees the same edits in the same order (sessions mutate the trees)
     reserve = (cfg0.max_inflight * max(cfg0.max_batch_rays, 3840 * 2160) * 24) + (1 << 30)
     t0 = time.time()
     trees, dtrees, refused, pool_bytes = [], [], 0, 0
@@ -237,29 +242,35 @@ def served_bench(args, capacity_mrays: float) -> dict:
         pool_bytes += d.device_bytes
     log(f"served: {len(trees)} trees resident ({pool_bytes / 2**30:.2f} GiB) in "
         f"{time.time() - t0:.1f}s, {refused} refused")
+    pools = {modes[0]: (trees, dtrees)}
+    for mode in modes[1:]:
+        copies = [t.clone() for t in trees]
+        pools[mode] = (copies, [serve.admit(t, reserve_bytes=reserve) for t in copies])
     sizes = [t.get_size() for t in trees]
     probe = serve.plan(serve.ServeConfig(qps=1000.0, duration_s=20.0, seed=args.seed), sizes)
     mean_rays = sum(r.width * r.height for r in probe if r.kind == 0) / len(probe)
     qps_cap = capacity_mrays * 1e6 / mean_rays
-    # Untimed warm-up session (the served counterpart of --warmup): the first
-    # edits of a tree grow its device arrays and fault in its host mirror, a
-    # one-time cost that would otherwise land on whichever mode runs first.
+    # Untimed warm-up session per pool (the served counterpart of --warmup):
+    # the first edits of a tree grow its device arrays and fault in its host
+    # mirror, a one-time cost that would otherwise land on one measured session.
     t0 = time.time()
-    serve.run(trees, dtrees, serve.ServeConfig(
-        qps=round(0.6 * qps_cap, 1), duration_s=min(1.0, args.served_seconds),
-        seed=args.seed + 7, max_inflight=args.served_inflight,
-        max_batch_rays=args.served_batch_rays))
-    log(f"served: warm-up session in {time.time() - t0:.1f}s")
+    for mode, (ptrees, pdtrees) in pools.items():
+        serve.run(ptrees, pdtrees, serve.ServeConfig(
+            qps=round(0.6 * qps_cap, 1), duration_s=min(1.0, args.served_seconds),
+            seed=args.seed + 7, mode=mode, max_inflight=args.served_inflight,
+            max_batch_rays=args.served_batch_rays))
+    log(f"served: warm-up sessions in {time.time() - t0:.1f}s")
     levels = []
     for frac in args.served_loads:
         qps = round(frac * qps_cap, 1)
         row = {"load": frac, "qps": qps}
-        for mode in ("batched", "plain"):
+        for mode in modes:
+            ptrees, pdtrees = pools[mode]
             cfg = serve.ServeConfig(qps=qps, duration_s=args.served_seconds, seed=args.seed,
                                     mode=mode, max_inflight=args.served_inflight,
                                     max_batch_rays=args.served_batch_rays)
-            # sessions run back to back on the same pool; their sparse edits accumulate
-            r = serve.run(trees, dtrees, cfg)
+            # a mode's sessions run back to back on its pool; their edits accumulate
+            r = serve.run(ptrees, pdtrees, cfg)
             row[mode] = {"mrays_per_s": round(r["mrays_per_s"], 1),
                          "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
                          "render_p99_ms": round(r["render_p99_ms"], 3),
@@ -269,6 +280,7 @@ def served_bench(args, capacity_mrays: float) -> dict:
                          "requests": int(r["requests"]), "edits": int(r["edits"]),
                          "launches": int(r["launches"]),
                          "jobs_per_launch": round(r["mean_jobs_per_launch"], 2),
+                         "single_launches": int(r["single_launches"]),
                          "elapsed_s": round(r["elapsed_s"], 3)}
             log(f"served load {frac} ({qps} req/s) {mode}: {row[mode]}")
         b, p = row["batched"], row["plain"]
@@ -282,7 +294,8 @@ def served_bench(args, capacity_mrays: float) -> dict:
         "mean_rays_per_request": round(mean_rays), "capacity_qps": round(qps_cap, 1),
         "max_inflight": args.served_inflight, "max_batch_rays": args.served_batch_rays,
         "batch_variant": batch_variants()[current_batch_variant()],
-        "arrival_seconds": args.served_seconds, "warmup_session": "1 s at load 0.6, batched",
+        "arrival_seconds": args.served_seconds,
+        "warmup_session": "1 s at load 0.6 per mode, each mode on its own copy of the pool",
         "levels": levels,
     }
 

@@ -60,6 +60,9 @@ typedef struct svo_entry {
 /* Octree::new (src/octree/mod.rs:173-205) */
 int svo_octree_new(uint32_t size, uint32_t brick_dim, svo_octree** out);
 void svo_octree_free(svo_octree* tree);
+/* Octree::clone (#[derive(Clone)], octree/types.rs:167-168): an independent
+ * copy with the same nodes, palettes, MIPs and strategy */
+int svo_octree_clone(const svo_octree* tree, svo_octree** out);
 /* Octree::get_size (src/octree/mod.rs:374-376) */
 uint32_t svo_octree_get_size(const svo_octree* tree);
 uint32_t svo_octree_get_brick_dim(const svo_octree* tree);
@@ -387,8 +390,12 @@ int svo_current_batch_variant(void);
 /*    that run in consecutive launches, so requests that arrive meanwhile   */
 /*    join the next launch; up to max_inflight launches are queued on their */
 /*    own streams.                                                          */
+/*    A launch that would hold a single whole request runs as a plain       */
+/*    render instead (the persistent kernel's set-up buys nothing for one   */
+/*    job), so a lightly loaded loop behaves as mode 1.                     */
 /*  mode 1, per-request plain launches: each render request is one          */
 /*    svo_render launch, in arrival order, on up to max_inflight streams.  */
+/*  mode 2: mode 0 with the persistent batched kernel for every launch.    */
 /* Both modes keep per-tree order: a render sees exactly the edits that     */
 /* arrived before it. Edits run on a host worker thread (host octree edit,  */
 /* then svo_tree_update on its own stream) once no earlier render of the    */
@@ -401,7 +408,8 @@ typedef struct svo_serve_config {
   double qps;               /* mean arrival rate, requests per second */
   double duration_s;        /* arrivals are generated over [0, duration_s) */
   uint64_t seed;
-  uint32_t mode;            /* 0 continuous batching, 1 per-request launches */
+  uint32_t mode;            /* 0 continuous batching, 1 per-request launches,
+                               2 continuous batching, batched kernel only */
   uint32_t max_inflight;    /* launches queued at once (1..16) */
   uint64_t max_batch_rays;  /* ray budget of one batched launch (mode 0) */
   uint32_t max_batch_jobs;  /* chunk budget of one batched launch (mode 0) */
@@ -443,6 +451,8 @@ typedef struct svo_serve_stats {
    * tree to go idle), host octree edit, layout update and device upload */
   double edit_wait_p99_ms, edit_apply_p99_ms, edit_upload_p99_ms;
   uint64_t timed_out;       /* requests dropped by the deadline (latency -2) */
+  uint64_t single_launches; /* mode 0: launches that held one whole request and ran as a
+                               plain render */
 } svo_serve_stats;
 
 /* The request sequence of a serving session (deterministic in cfg and the

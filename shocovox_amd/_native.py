@@ -97,7 +97,7 @@ class ServeStats(C.Structure):
                 + [("edit_records", C.c_uint64), ("edit_bricks", C.c_uint64)]
                 + [(k, C.c_double) for k in ("edit_wait_p99_ms", "edit_apply_p99_ms",
                                              "edit_upload_p99_ms")]
-                + [("timed_out", C.c_uint64)])
+                + [("timed_out", C.c_uint64), ("single_launches", C.c_uint64)])
 
 
 _P = C.c_void_p
@@ -135,6 +135,7 @@ _SIGS = {
     "svo_octree_from_bytes": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "svo_octree_save": (C.c_int, [_P, C.c_char_p]),
     "svo_octree_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "svo_octree_clone": (C.c_int, [_P, C.POINTER(_P)]),
     "svo_octree_from_vox_bytes": (C.c_int, [_P, C.c_size_t, C.c_uint32, C.POINTER(_P)]),
     "svo_octree_load_vox": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     "svo_tree_load": (C.c_int, [C.c_char_p, C.POINTER(_P), C.POINTER(_P)]),

@@ -106,6 +106,13 @@ int svo_octree_new(uint32_t size, uint32_t brick_dim, svo_octree** out) {
 
 void svo_octree_free(svo_octree* tree) { delete tree; }
 
+int svo_octree_clone(const svo_octree* tree, svo_octree** out) {
+  if (!tree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
+  *out = new (std::nothrow) svo_octree{tree->tree};
+  if (!*out) return fail(SVO_INVALID_ARGUMENT, "allocation failed");
+  return SVO_OK;
+}
+
 uint32_t svo_octree_get_size(const svo_octree* tree) { return tree ? tree->tree.size() : 0; }
 uint32_t svo_octree_get_brick_dim(const svo_octree* tree) {
   return tree ? tree->tree.brick_dim() : 0;

@@ -115,12 +115,12 @@ int check_config(const svo_serve_config* cfg, uint32_t n_trees) {
     return serve_fail(SVO_INVALID_ARGUMENT, "edit_fraction outside [0, 1]");
   if (cfg->edit_fraction > 0.0 && cfg->max_edit_voxels == 0)
     return serve_fail(SVO_INVALID_ARGUMENT, "max_edit_voxels is 0");
-  if (cfg->mode > 1) return serve_fail(SVO_INVALID_ARGUMENT, "mode must be 0 or 1");
+  if (cfg->mode > 2) return serve_fail(SVO_INVALID_ARGUMENT, "mode must be 0, 1 or 2");
   if (!(cfg->deadline_ms >= 0.0) || !std::isfinite(cfg->deadline_ms))
     return serve_fail(SVO_INVALID_ARGUMENT, "deadline_ms must be >= 0");
   if (cfg->max_inflight == 0 || cfg->max_inflight > 16)
     return serve_fail(SVO_INVALID_ARGUMENT, "max_inflight outside 1..16");
-  if (cfg->mode == 0 && (cfg->max_batch_rays < 64 || cfg->max_batch_rays > (1ull << 30) ||
+  if (cfg->mode != 1 && (cfg->max_batch_rays < 64 || cfg->max_batch_rays > (1ull << 30) ||
                          cfg->max_batch_jobs == 0 || cfg->max_batch_jobs > 4096))
     return serve_fail(SVO_INVALID_ARGUMENT, "batch budget outside 64..2^30 rays, 1..4096 jobs");
   return SVO_OK;
@@ -406,9 +406,17 @@ int alloc_slot(Slot& sl, uint64_t cap, uint32_t digest_cap, uint32_t batch_jobs)
   return SVO_OK;
 }
 
-// Launches the slot's chunks (already in sl.chunks) and records its event.
-int launch_slot(Session& s, Slot& sl) {
-  const bool batched = s.cfg.mode == 0;
+// Launches the slot's chunks (already in sl.chunks) and records its event;
+// *single is set when a mode-0 launch ran as one plain render.
+int launch_slot(Session& s, Slot& sl, bool* single) {
+  // Mode 0 runs a launch that holds one whole request as a plain render (the
+  // persistent kernel's job table, counters and graph buy nothing for one
+  // job); mode 2 uses the persistent kernel for every launch.
+  const Chunk& c0 = sl.chunks[0];
+  const bool whole = sl.chunks.size() == 1 && c0.first_tile == 0 &&
+                     c0.tiles == tiles_of(s.plan[c0.req]);
+  const bool batched = s.cfg.mode == 2 || (s.cfg.mode == 0 && !whole);
+  *single = s.cfg.mode == 0 && whole;
   std::string err;
   if (batched) {
     std::vector<BatchJobIn> jobs(sl.chunks.size());
@@ -501,7 +509,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
                 uint64_t cap) {
   const svo_serve_config& cfg = s.cfg;
   const size_t n = s.plan.size();
-  const bool batched = cfg.mode == 0;
+  const bool batched = cfg.mode != 1;
 
   // output pools: a batched slot holds max_batch_rays (whole tiles); a plain
   // slot holds the largest request of the plan
@@ -537,7 +545,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   std::vector<uint64_t> take;
   std::deque<uint32_t> queue;
   size_t next_arrival = 0, done = 0;
-  uint64_t launches = 0, chunks_launched = 0, rays_done = 0;
+  uint64_t launches = 0, chunks_launched = 0, rays_done = 0, single_launches = 0;
 
   // every tree that an edit of the plan touches gets its patch staging now,
   // so no edit allocates pinned or device memory inside the session
@@ -716,7 +724,9 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
           ++it;
       }
       if (free && !free->chunks.empty()) {
-        status = launch_slot(s, *free);
+        bool single = false;
+        status = launch_slot(s, *free, &single);
+        single_launches += single ? 1u : 0u;
         ++launches;
         chunks_launched += free->chunks.size();
         changed = true;  // another slot may be free for the rest of the queue
@@ -792,6 +802,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
     o.timed_out = ntimed;
     o.rays = rays_done;
     o.launches = launches;
+    o.single_launches = single_launches;
     o.elapsed_s = last - first;
     o.mrays_per_s = o.elapsed_s > 0 ? static_cast<double>(rays_done) / o.elapsed_s / 1e6 : 0.0;
     o.p50_ms = percentile(all, 0.5);

@@ -210,6 +210,12 @@ class Octree:
         _check(N.lib().svo_octree_load(str(path).encode(), C.byref(h)))
         return cls(0, 0, _handle=h.value)
 
+    def clone(self) -> "Octree":
+        """An independent copy (the reference's Octree derives Clone)."""
+        h = C.c_void_p()
+        _check(N.lib().svo_octree_clone(self._h, C.byref(h)))
+        return Octree(0, 0, _handle=h.value)
+
     @classmethod
     def load_vox_file(cls, filename: str, brick_dimension: int) -> "Octree":
         """Octree::load_vox_file (convert/magicavoxel.rs:259-278): a MagicaVoxel

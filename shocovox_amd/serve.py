@@ -20,7 +20,10 @@ from .octree import DeviceTree, Octree, Viewport, _check
 # resolutions of the served workload, most popular first
 SERVED_RESOLUTIONS = ((256, 256), (1280, 720), (1920, 1080), (3840, 2160))
 
-MODES = {"batched": 0, "plain": 1}
+# batched: continuous batching (a launch holding one whole request runs as a
+# plain render); plain: one launch per request; batched-kernel: continuous
+# batching with the persistent batched kernel for every launch
+MODES = {"batched": 0, "plain": 1, "batched-kernel": 2}
 
 
 @dataclass

@@ -72,7 +72,7 @@ def test_served_results_same_alone_and_in_mix(gpu_device, load):
     _say(f"serve plan ({load}): {len(plan)} requests, {n_edit} edits")
     assert n_edit > 5 and len(plan) > 100
 
-    a, b, c = _trees(), _trees(), _trees()
+    a, b, c, k = _trees(), _trees(), _trees(), _trees()
     _say("serve: batched session")
     ra = serve.run(a, [DeviceTree(t) for t in a], cfg, digests=True)
     _say(f"batched: {ra['launches']} launches, {ra['mean_jobs_per_launch']:.2f} chunks/launch, "
@@ -81,7 +81,14 @@ def test_served_results_same_alone_and_in_mix(gpu_device, load):
     rb = serve.run(b, [DeviceTree(t) for t in b], dataclasses.replace(cfg, mode="plain"),
                    digests=True)
     _say(f"plain: {rb['launches']} launches, p50 {rb['p50_ms']:.2f} ms p99 {rb['p99_ms']:.2f} ms")
-    for r in (ra, rb):
+    _say("serve: batched-kernel session")
+    rk = serve.run(k, [DeviceTree(t) for t in k], dataclasses.replace(cfg, mode="batched-kernel"),
+                   digests=True)
+    _say(f"batched-kernel: {rk['launches']} launches, single {ra['single_launches']} in mode 0")
+    assert rk["single_launches"] == 0 and rb["single_launches"] == 0
+    if load == "steady":
+        assert ra["single_launches"] > 0
+    for r in (ra, rb, rk):
         assert r["requests"] == len(plan) and r["rejected"] == 0
         assert r["renders"] + r["edits"] == len(plan) and r["edits"] == n_edit
         assert r["rays"] == sum(q.width * q.height for q in plan if q.kind == 0)
@@ -92,6 +99,7 @@ def test_served_results_same_alone_and_in_mix(gpu_device, load):
     if load == "burst":
         assert ra["mean_jobs_per_launch"] > 1.5
     assert np.array_equal(ra["digests"], rb["digests"])
+    assert np.array_equal(ra["digests"], rk["digests"])
     assert all(int(d) != 0 for d, q in zip(ra["digests"], plan) if q.kind == 0)
 
     _say("serve: oracle replay")
@@ -101,6 +109,7 @@ def test_served_results_same_alone_and_in_mix(gpu_device, load):
     assert not bad, f"{len(bad)} requests differ from the oracle replay, first {bad[:5]}"
     for i in range(len(TREES)):
         assert _models_equal(a[i], c[i]) and _models_equal(b[i], c[i]), i
+        assert _models_equal(k[i], c[i]), i
 
 
 def test_served_queue_limit_rejects_and_accounts(gpu_device):

@@ -441,3 +441,16 @@ def test_insert_many_matches_single_inserts():
     for p in xyz[:500]:
         b.clear(tuple(int(v) for v in p))
     assert np.array_equal(a.get_many(allp), b.get_many(allp))
+
+
+def test_clone_is_equal_and_independent():
+    """Octree::clone: the same model and bytes; edits of the copy leave the
+    original as it was."""
+    t = Octree.scene(2, 64, 4, 5)
+    t.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    c = t.clone()
+    assert c.to_bytes() == t.to_bytes()
+    before = t.to_bytes()
+    c.insert((1, 2, 3), OctreeEntry.visual(0x12345678))
+    c.clear((40, 40, 40))
+    assert t.to_bytes() == before and c.to_bytes() != before
