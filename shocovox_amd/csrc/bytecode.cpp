@@ -215,9 +215,17 @@ bool read_children(Reader& r, NodeChildren* c) {
 }
 
 void put_int(std::string* o, uint64_t v) {
-  o->push_back('i');
-  o->append(std::to_string(v));
-  o->push_back('e');
+  // "i<decimal>e" formatted in place (no temporary string per integer)
+  char buf[24];
+  char* e = buf + sizeof(buf);
+  char* p = e;
+  *--p = 'e';
+  do {
+    *--p = static_cast<char>('0' + v % 10);
+    v /= 10;
+  } while (v);
+  *--p = 'i';
+  o->append(p, static_cast<size_t>(e - p));
 }
 void put_str(std::string* o, const char* s) {
   const size_t l = std::strlen(s);
@@ -395,6 +403,18 @@ bool octree_from_bencode(const uint8_t* data, size_t n, Octree* out, std::string
 void octree_to_bencode(const Octree& t, std::string* out) {
   std::string& o = *out;
   o.clear();
+  {
+    // about 4 characters per stored value: reserve once instead of regrowing
+    size_t values = 64;
+    const NodePool& pl = t.nodes();
+    for (uint32_t k = 0; k < pl.len(); ++k) {
+      const NodeContent& n = pl.get(k);
+      const int nb = n.kind == NodeKind::Leaf ? 8 : (n.kind == NodeKind::UniformLeaf ? 1 : 0);
+      for (int b = 0; b < nb; ++b) values += 4 + n.bricks[b].parted.size();
+      values += 4 + t.node_mip(k).parted.size() + 12;
+    }
+    o.reserve(values * 5);
+  }
   o.push_back('l');
   put_int(&o, t.auto_simplify ? 1 : 0);
   put_int(&o, t.size());

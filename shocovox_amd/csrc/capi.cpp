@@ -121,6 +121,7 @@ uint32_t svo_octree_get_brick_dim(const svo_octree* tree) {
 int svo_octree_set_auto_simplify(svo_octree* tree, int enabled) {
   if (!tree) return fail(SVO_INVALID_ARGUMENT, "tree is NULL");
   tree->tree.auto_simplify = enabled != 0;
+  tree->bytes_valid = false;
   return SVO_OK;
 }
 
@@ -395,12 +396,21 @@ int svo_tree_from_octree(svo_octree* tree, svo_tree** out) {
 
 int svo_octree_to_bytes(const svo_octree* tree, uint8_t* out, size_t cap, size_t* len) {
   if (!tree || !len) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
-  std::string b;
-  svo::octree_to_bencode(tree->tree, &b);
+  // The size query and the copy that follows it encode the tree once: the
+  // encoding is kept on the handle until the tree changes.
+  if (!tree->bytes_valid || tree->bytes_epoch != tree->tree.edit_epoch()) {
+    svo::octree_to_bencode(tree->tree, &tree->bytes);
+    tree->bytes_epoch = tree->tree.edit_epoch();
+    tree->bytes_valid = true;
+  }
+  const std::string& b = tree->bytes;
   *len = b.size();
   if (out) {
     if (cap < b.size()) return fail(SVO_INVALID_ARGUMENT, "output buffer too small");
     std::memcpy(out, b.data(), b.size());
+    tree->bytes.clear();
+    tree->bytes.shrink_to_fit();
+    tree->bytes_valid = false;
   }
   return SVO_OK;
 }

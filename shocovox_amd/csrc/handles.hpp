@@ -11,6 +11,10 @@
 
 struct svo_octree {
   svo::Octree tree;
+  // encoding of the last svo_octree_to_bytes size query (see there)
+  mutable std::string bytes;
+  mutable uint64_t bytes_epoch = 0;
+  mutable bool bytes_valid = false;
 };
 
 struct svo_flat {

@@ -1410,14 +1410,19 @@ void Octree::set_mip_method(uint32_t level, MipMethodSpec m) {
   if (m.kind == MipMethod::Posterize || m.kind == MipMethod::PosterizeBD)
     m.thr = std::min(std::max(m.thr, 0.0f), 1.0f);
   mip_strategy_.methods[level] = m;
+  ++edit_epoch_;
 }
 
 void Octree::set_mip_color_threshold(uint32_t level, float thr) {
   // set_color_similarity_thr_internal (mipmap.rs:618-631)
   mip_strategy_.color_thresholds[level] = std::min(std::max(thr, 0.0f), 1.0f);
+  ++edit_epoch_;
 }
 
-void Octree::reset_mip_strategy() { mip_strategy_ = MipStrategy{}; }
+void Octree::reset_mip_strategy() {
+  mip_strategy_ = MipStrategy{};
+  ++edit_epoch_;
+}
 
 void Octree::assign_mips(std::vector<Brick> mips, MipStrategy strategy) {
   node_mips_ = std::move(mips);
@@ -1437,6 +1442,7 @@ void Octree::switch_albedo_mip_maps(bool enabled) {
 
 // recalculate_mips (mipmap.rs:798-862): depth first, children before parents
 void Octree::recalculate_mips() {
+  ++edit_epoch_;
   node_mips_.assign(nodes_.len() ? nodes_.len() : 1, Brick{});
   for (uint32_t k = 0; k < nodes_.len(); ++k) nodes_.touch(k);
   struct Item {

@@ -240,7 +240,7 @@ class Octree:
         _check(N.lib().svo_octree_to_bytes(self._h, None, 0, C.byref(n)))
         buf = (C.c_uint8 * max(1, n.value))()
         _check(N.lib().svo_octree_to_bytes(self._h, buf, n.value, C.byref(n)))
-        return bytes(buf[:n.value])
+        return C.string_at(buf, n.value)
 
     def save(self, path: str) -> None:
         """Octree::save (octree/mod.rs:151-157)."""
