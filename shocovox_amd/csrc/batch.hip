@@ -140,7 +140,7 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
         d = V3f{rr[3], rr[4], rr[5]};
       }
       if (valid) {
-        const CastResult r = cast_ray(tv, o, d);
+        const CastResult r = cast_ray<true>(tv, o, d);
         store_result(tv, job.out, idx, r, o);
       }
     }

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __re
   const float* r = rays + 6ull * i;
   const V3f o{r[0], r[1], r[2]};
   const V3f d{r[3], r[4], r[5]};
-  const CastResult res = cast_ray(t, o, d);
+  const CastResult res = cast_ray<true>(t, o, d);
   store_result(t, out, i, res, o);
 }
 
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void cast_lod_kernel(TreeView t, const float* 
   const float* r = rays + 6ull * i;
   const V3f o{r[0], r[1], r[2]};
   const V3f d{r[3], r[4], r[5]};
-  const CastResult res = cast_ray_lod(t, t.ray_lut, o, d, viewing_distance);
+  const CastResult res = cast_ray_lod<true>(t, t.ray_lut, o, d, viewing_distance);
   store_result(t, out, i, res, o);
 }
 
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void render_lod_kernel(TreeView t, Camera cam,
   const uint32_t y = (blockIdx.y * 2u + (wave >> 1)) * 8u + (lane / 8u);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
-  const CastResult res = cast_ray_lod(t, t.ray_lut, cam.origin, d, viewing_distance);
+  const CastResult res = cast_ray_lod<true>(t, t.ray_lut, cam.origin, d, viewing_distance);
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
@@ -103,7 +103,7 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
   const uint32_t y = (by * 2u + (wave >> 1)) * TH + (lane / TW);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
-  const CastResult res = cast_ray(t, lut, cam.origin, d, s_nodes, ntop);
+  const CastResult res = cast_ray<true>(t, lut, cam.origin, d, s_nodes, ntop);
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void shade_kernel(TreeView t, ShadeParams sp, 
     o = glass;
     d = normalized(glass - sp.origin);
   }
-  const CastResult r = cast_ray(t, o, d);
+  const CastResult r = cast_ray<true>(t, o, d);
   uint32_t rgba;
   if (r.hit) {
     const uint32_t ci = pix_color_index(r.pix);
@@ -303,9 +303,8 @@ bool validate_desc(const svo_tree_desc& d, std::string* err) {
     *err = "tree has no nodes";
     return false;
   }
-  auto pow2 = [](uint32_t v) { return v != 0 && (v & (v - 1)) == 0; };
-  if (!pow2(d.brick_dim) || !pow2(d.octree_size) || d.octree_size < 2 * d.brick_dim ||
-      d.octree_size > (1u << 24)) {
+  // exact_pow2_sizes is what lets every kernel run the traversal's P2 form
+  if (!exact_pow2_sizes(d.octree_size, d.brick_dim) || d.octree_size < 2 * d.brick_dim) {
     *err = "octree size and brick dimension must be powers of two, size >= 2 * dim, size <= 2^24";
     return false;
   }

@@ -62,6 +62,14 @@ SVO_HD inline float rsignum(float x) {
 SVO_HD inline int32_t step_int(float s) { return s == 1.0f ? 1 : (s == -1.0f ? -1 : 0); }
 
 SVO_HD inline int32_t f2i32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // v_cvt_i32_f32 truncates toward zero, clamps out-of-range values to
+  // [INT32_MIN, INT32_MAX] and converts NaN to 0: Rust's `as i32` in one
+  // instruction (the portable form below compiles to compares and branches)
+  int32_t r;
+  __asm__("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+#endif
   if (x != x) return 0;
   if (x >= 2147483648.0f) return 2147483647;
   if (x < -2147483648.0f) return (-2147483647 - 1);
@@ -77,6 +85,12 @@ SVO_HD inline uint64_t f2usize(float x) {
 
 // `f as u32` (saturating, NaN -> 0)
 SVO_HD inline uint32_t f2u32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // v_cvt_u32_f32: truncation, clamping to [0, UINT32_MAX], NaN -> 0
+  uint32_t r;
+  __asm__("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+#endif
   if (!(x > 0.0f)) return 0;
   if (x >= 4294967296.0f) return 0xFFFFFFFFu;
   return static_cast<uint32_t>(x);

@@ -43,6 +43,59 @@ struct TreeView {
   uint32_t mips_enabled;  // MIPMapStrategy::enabled: LOD queries probe node MIPs
 };
 
+// Every bounds size a traversal meets is the tree size halved k times (PUSH)
+// or doubled back (POP), never below the brick dimension, and a brick's
+// voxel size is a bounds size over the brick dimension. When the tree size
+// is a power of two no larger than 2^24 and the brick dimension a power of
+// two, all of these are powers of two in [1, 2^24] and all bounds
+// coordinates are integers below 2^24. Then x / s equals x * (1/s) exactly
+// and the POP's a - fmodf(a, b) equals floor(a / b) * b exactly (see
+// pow2_rcp and sub_fmod), so the traversal can be compiled with P2 = true,
+// without the general division and fmodf fallbacks; it is the same
+// arithmetic, bit for bit. (Octree::new checks both powers through f32
+// logarithms, which can admit a non-power at large sizes; such trees take
+// the P2 = false form; the device upload refuses them.)
+SVO_HD inline bool exact_pow2_sizes(uint64_t size, uint64_t dim) {
+  return size != 0 && (size & (size - 1)) == 0 && size <= (1ull << 24) && dim != 0 &&
+         (dim & (dim - 1)) == 0 && dim <= size;
+}
+
+// 1 / s for a bounds size s (a power of two in [1, 2^24] when P2): the
+// exponent is negated by one integer subtraction
+template <bool P2>
+SVO_HD inline float size_rcp(float s) {
+  if constexpr (P2) {
+    return __builtin_bit_cast(float, 0x7F000000u - __builtin_bit_cast(uint32_t, s));
+  } else {
+    return pow2_rcp(s);
+  }
+}
+template <bool P2>
+SVO_HD inline float div_size(float x, float s, float rcp) {
+  if constexpr (P2) {
+    return x * rcp;
+  } else {
+    return div_by(x, s, rcp);
+  }
+}
+template <bool P2>
+SVO_HD inline V3f div3_size(V3f v, float s, float rcp) {
+  if constexpr (P2) {
+    return V3f{v.x * rcp, v.y * rcp, v.z * rcp};
+  } else {
+    return div3_by(v, s, rcp);
+  }
+}
+// a - fmodf(a, b) for a bounds coordinate a >= +0 and b twice a bounds size
+template <bool P2>
+SVO_HD inline float sub_fmod_size(float a, float b, float rcp_b) {
+  if constexpr (P2) {
+    return floorf(a * rcp_b) * b;
+  } else {
+    return sub_fmod(a, b, rcp_b);
+  }
+}
+
 struct CastResult {
   uint32_t hit;
   uint32_t pix;
@@ -77,6 +130,7 @@ SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
 }
 
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
+template <bool P2>
 SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t, uint32_t form,
                                                               uint32_t word, Cube bb, V3f d,
                                                               V3f sig, V3f scale, V3f* p,
@@ -90,15 +144,15 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   }
   const int dim = static_cast<int>(t.dim);
   const float fdim = static_cast<float>(t.dim);
-  const float rcp_size = pow2_rcp(bb.size);
+  const float rcp_size = size_rcp<P2>(bb.size);
   const V3f rel = (*p - bb.min) * fdim;
-  const V3f pib = div3_by(rel, bb.size, rcp_size);
+  const V3f pib = div3_size<P2>(rel, bb.size, rcp_size);
   auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
   int ix = clampi(f2i32(pib.x));
   int iy = clampi(f2i32(pib.y));
   int iz = clampi(f2i32(pib.z));
   int flat = ix + iy * dim + iz * dim * dim;
-  const float unit = div_by(bb.size, fdim, t.inv_dim);
+  const float unit = div_size<P2>(bb.size, fdim, t.inv_dim);
   Cube cb{bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
                        unit,
           unit};
@@ -112,7 +166,8 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
       const V3f scaled = V3f{static_cast<float>(ix), static_cast<float>(iy),
                              static_cast<float>(iz)} *
                          bb.size;
-      const Cube hb{bb.min + div3_by(scaled, fdim, t.inv_dim), div_by(bb.size, fdim, t.inv_dim)};
+      const Cube hb{bb.min + div3_size<P2>(scaled, fdim, t.inv_dim),
+                    div_size<P2>(bb.size, fdim, t.inv_dim)};
       r->impact = *p;
       r->cell = hb;
       return true;
@@ -132,7 +187,8 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
 // passes). `lut` is the RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the
 // caller staged it. `top` optionally holds a copy of the first `ntop` node
 // records (the top levels, as records are numbered breadth-first), e.g.
-// staged in LDS; those records are read from it instead of t.nodes.
+// staged in LDS; those records are read from it instead of t.nodes. P2
+// requires exact_pow2_sizes(t.size, t.dim).
 //
 // With LOD and the tree's MIPs enabled, every stack-loop iteration first
 // tests the node's level against the distance travelled (:368-386): the
@@ -140,7 +196,7 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
 // POP and is not reset by a restart from the root; when the level is below
 // |origin - round(p / (2 level)) * 2 level| / viewing_distance, the node's
 // MIP brick is probed over the node's bounds like a uniform leaf's brick.
-template <bool LOD, class LutT>
+template <bool LOD, bool P2, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop, float viewing_distance) {
@@ -200,20 +256,20 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
         const V3f snapped{roundf(p.x / grid) * grid, roundf(p.y / grid) * grid,
                           roundf(p.z / grid) * grid};
         if (mip_level < length(o - snapped) / viewing_distance &&
-            probe_brick(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r))
+            probe_brick<P2>(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r))
           goto hit_found;
       }
       if (target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
         const bool uniform = kind == KIND_UNIFORM_LEAF;
         const uint32_t slot = uniform ? 0u : target;
-        if (probe_brick(t, brick_form(meta, slot), n.child[slot],
-                        uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale, &p,
-                        &r))
+        if (probe_brick<P2>(t, brick_form(meta, slot), n.child[slot],
+                            uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
+                            &p, &r))
           goto hit_found;
       }
-      const float rcp_size = pow2_rcp(bounds.size);
+      const float rcp_size = size_rcp<P2>(bounds.size);
       const V3f rel = (p - bounds.min) * 4.0f;
-      V3f bp = div3_by(rel, bounds.size, rcp_size);
+      V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
       bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
       uint32_t flat = bitmap_index(bp, &r.flags);
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
@@ -225,9 +281,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
         if (count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);
           const float s2 = bounds.size * 2.0f;
-          const float rcp_s2 = pow2_rcp(s2);
-          const V3f pmin{sub_fmod(bounds.min.x, s2, rcp_s2), sub_fmod(bounds.min.y, s2, rcp_s2),
-                         sub_fmod(bounds.min.z, s2, rcp_s2)};
+          const float rcp_s2 = size_rcp<P2>(s2);
+          const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
+                         sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
+                         sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
           const uint32_t from = hash_region(center - pmin, bounds.size);
           const V3f st = dda_step(d, sig, &p, bounds, scale);
           target = step_octant(from, st);
@@ -262,7 +319,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
           if (target == OOB_OCTANT) break;
           tb = child_bounds_for(bounds, target);
           const V3f sd = st * 4.0f;
-          bp = bp + div3_by(sd, bounds.size, rcp_size);
+          bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
           flat = bitmap_index(bp, &r.flags);
           // stop at a reachable occupied child or a non-empty leaf brick; the
           // tests run cheapest first and the child word is read last
@@ -288,24 +345,25 @@ hit_found:
   return r;
 }
 
-template <class LutT>
+template <bool P2 = false, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t,
                                                                 const LutT* __restrict__ lut,
                                                                 V3f o, V3f d,
                                                                 const NodeRec* top = nullptr,
                                                                 uint32_t ntop = 0u) {
-  return cast_ray_impl<false>(t, lut, o, d, top, ntop, 0.0f);
+  return cast_ray_impl<false, P2>(t, lut, o, d, top, ntop, 0.0f);
 }
 
-template <class LutT>
+template <bool P2 = false, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_lod(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, float viewing_distance) {
-  return cast_ray_impl<true>(t, lut, o, d, nullptr, 0u, viewing_distance);
+  return cast_ray_impl<true, P2>(t, lut, o, d, nullptr, 0u, viewing_distance);
 }
 
+template <bool P2 = false>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView& t, V3f o,
                                                                 V3f d) {
-  return cast_ray(t, t.ray_lut, o, d);
+  return cast_ray<P2>(t, t.ray_lut, o, d);
 }
 
 // The same traversal as cast_ray in resumable form: ray_begin sets a ray up,
@@ -391,9 +449,9 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
     const bool uniform = kind == KIND_UNIFORM_LEAF;
     const uint32_t slot = uniform ? 0u : s.target;
     CastResult hr = miss_result(0u);
-    if (probe_brick(t, brick_form(meta, slot), n.child[slot],
-                    uniform ? s.bounds : child_bounds_for(s.bounds, s.target), s.d, sig,
-                    s.scale, &s.p, &hr)) {
+    if (probe_brick<false>(t, brick_form(meta, slot), n.child[slot],
+                           uniform ? s.bounds : child_bounds_for(s.bounds, s.target), s.d, sig,
+                           s.scale, &s.p, &hr)) {
       r = hr;
       r.hit = 1;
       r.normal = cube_impact_normal(hr.cell, hr.impact);

@@ -31,7 +31,8 @@ def lib():
     if _lib is None:
         subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
         L = C.CDLL(str(LIB))
-        L.hs_cast.argtypes = [C.POINTER(_Tree), C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]
+        L.hs_cast.argtypes = [C.POINTER(_Tree), C.c_void_p, C.c_uint64, C.c_void_p, C.c_int,
+                              C.c_int]
         L.hs_viewport_rays.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         _lib = L
     return _lib
@@ -45,9 +46,11 @@ def ray_lut() -> np.ndarray:
 
 
 def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False,
-         max_steps: int = 0, viewing_distance: float | None = None) -> np.ndarray:
+         max_steps: int = 0, viewing_distance: float | None = None,
+         general: bool = False) -> np.ndarray:
     """The product traversal on the host: cast_ray, with resumable=True its
-    ray_begin / ray_step form, with a viewing distance cast_ray_lod."""
+    ray_begin / ray_step form, with a viewing distance cast_ray_lod. As on the
+    device, power-of-two trees take the P2 form unless general=True."""
     a = flat_arrays
     keep = [np.ascontiguousarray(a["nodes"]), np.ascontiguousarray(a["brick_pix"]),
             np.ascontiguousarray(a["brick_occ"]), np.ascontiguousarray(a["colors"]), ray_lut()]
@@ -59,7 +62,7 @@ def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False,
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
     out = np.zeros(len(rays), HIT_DTYPE)
     mode = 2 if viewing_distance is not None else int(resumable)
-    lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, mode)
+    lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, mode, int(general))
     return out
 
 

@@ -33,8 +33,11 @@ struct hs_hit {
 
 // mode 0: cast_ray; mode 1: the resumable form (ray_begin, then ray_step
 // until the ray finishes), as the refill kernels run it; mode 2: cast_ray_lod
-// with t->viewing_distance.
-void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode) {
+// with t->viewing_distance. Modes 0 and 2 use the P2 form, as the kernels
+// do, when exact_pow2_sizes holds and `general` is 0, else the general form.
+void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode,
+             int general) {
+  const bool p2 = !general && svo::exact_pow2_sizes(static_cast<uint64_t>(t->size), t->dim);
   svo::TreeView v{t->nodes, t->brick_pix, t->brick_occ, t->colors, t->ray_lut, t->size, t->dim,
                   t->occ_words, t->color_count, svo::pow2_rcp(static_cast<float>(t->dim)), 0u,
                   t->max_steps ? t->max_steps : svo::MAX_TRAVERSAL_STEPS, t->mips_enabled};
@@ -43,9 +46,10 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int m
     const svo::V3f o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
     svo::CastResult c;
     if (mode == 0) {
-      c = svo::cast_ray(v, o, d);
+      c = p2 ? svo::cast_ray<true>(v, o, d) : svo::cast_ray<false>(v, o, d);
     } else if (mode == 2) {
-      c = svo::cast_ray_lod(v, v.ray_lut, o, d, t->viewing_distance);
+      c = p2 ? svo::cast_ray_lod<true>(v, v.ray_lut, o, d, t->viewing_distance)
+             : svo::cast_ray_lod<false>(v, v.ray_lut, o, d, t->viewing_distance);
     } else {
       svo::RayState st;
       if (svo::ray_begin(v, o, d, st)) {

@@ -278,6 +278,33 @@ def test_resumable_traversal_equals_cast_ray():
         assert all(v == 0 for k, v in r.items() if k != "rays"), r
 
 
+def test_p2_and_general_forms_agree():
+    """The kernels' P2 form (divisions by bounds sizes as multiplications by
+    exact reciprocals, the POP's fmod as floor * size) and the general form
+    give the same results, bit for bit, as each other and the oracle, for
+    plain and LOD traversal, on trees of every brick dimension 1..32."""
+    rng = np.random.default_rng(5)
+    for size, dim in [(8, 1), (16, 2), (64, 4), (128, 8), (128, 16), (256, 32)]:
+        t = Octree.new(size, dim)
+        n = min(size * size, 6000)
+        xyz = rng.integers(0, size, size=(n, 3)).astype(np.uint32)
+        ents = np.zeros((n, 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = (rng.integers(1, 8, size=n).astype(np.uint32) << 24) | 0xFF
+        t.insert_many(xyz, ents)
+        t.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+        a = t.flatten().arrays()
+        rays = W.incoherent_rays(size, 3000, seed=size + dim)
+        o = oracle.OracleTree(t.export_model())
+        for vd in (None, 2.0):
+            p2 = hostsim.cast(a, rays, viewing_distance=vd)
+            gen = hostsim.cast(a, rays, viewing_distance=vd, general=True)
+            assert p2.tobytes() == gen.tobytes(), (size, dim, vd)
+            exp = o.cast(rays) if vd is None else o.cast(rays, viewing_distance=vd)
+            r = compare(exp, {k: p2[k] for k in p2.dtype.names})
+            assert all(v == 0 for k, v in r.items() if k != "rays"), (size, dim, vd, r)
+
+
 def test_forced_iteration_guard_reports_and_ends_rays():
     """A lowered iteration bound (the guard the reference lacks) ends long rays
     as misses with flag bit 0 and leaves every other ray exactly as the
