@@ -68,8 +68,10 @@ __global__ __launch_bounds__(256) void render_lod_kernel(TreeView t, Camera cam,
 // breadth-first numbering) in LDS. XCD remaps the workgroup order so that
 // each of the 8 XCDs (which receive workgroups round-robin in dispatch
 // order) renders one contiguous band of the image, keeping that band's nodes
-// and bricks in its own L2.
-template <int MIN_WAVES, bool LDS_LUT, int TW, int TH, int NODES = 0, bool XCD = false>
+// and bricks in its own L2. WW runs the while-while arrangement of the
+// traversal (cast_ray_ww), which probes the due bricks of a wave together.
+template <int MIN_WAVES, bool LDS_LUT, int TW, int TH, int NODES = 0, bool XCD = false,
+          bool WW = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) {
   static_assert(TW * TH == 64, "a wave covers 64 pixels");
@@ -103,7 +105,8 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
   const uint32_t y = (by * 2u + (wave >> 1)) * TH + (lane / TW);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
-  const CastResult res = cast_ray<true>(t, lut, cam.origin, d, s_nodes, ntop);
+  const CastResult res = WW ? cast_ray_ww<true>(t, lut, cam.origin, d, s_nodes, ntop)
+                            : cast_ray<true>(t, lut, cam.origin, d, s_nodes, ntop);
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
@@ -229,6 +232,10 @@ const RenderVariant kRenderVariants[] = {
     {"w6-glut-8x8-n73", render_kernel<6, false, 8, 8, 73>, 8, 8},
     {"w6-slut-8x8-n73", render_kernel<6, true, 8, 8, 73>, 8, 8},
     {"w6-glut-8x8-xcd", render_kernel<6, false, 8, 8, 0, true>, 8, 8},
+    {"w1-glut-8x8-ww", render_kernel<1, false, 8, 8, 0, false, true>, 8, 8},
+    {"w5-glut-8x8-ww", render_kernel<5, false, 8, 8, 0, false, true>, 8, 8},
+    {"w6-slut-8x8-ww", render_kernel<6, true, 8, 8, 0, false, true>, 8, 8},
+    {"w5-slut-8x8-n73-ww", render_kernel<5, true, 8, 8, 73, false, true>, 8, 8},
     // lane refill measured 3.5x slower than one tile per wave on coherent
     // 1080p primary rays (lanes drift apart and lose the shared node lines);
     // kept as a variant for divergent workloads

@@ -249,6 +249,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       const NodeRec& n = (key < ntop ? top : t.nodes)[key];
       const uint64_t occ = n.occ;
       const uint32_t meta = n.meta;
+      // the child word of the current target, read together with the header
+      // (same record, one round trip); with target OOB_OCTANT it is child[0]
+      // and unused
+      const uint32_t tword = n.child[target & 7u];
       const uint32_t kind = node_kind(meta);
       bool backtrack = kind == KIND_UNIFORM_LEAF;
       if (LOD && t.mips_enabled) {
@@ -262,7 +266,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       if (target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
         const bool uniform = kind == KIND_UNIFORM_LEAF;
         const uint32_t slot = uniform ? 0u : target;
-        if (probe_brick<P2>(t, brick_form(meta, slot), n.child[slot],
+        if (probe_brick<P2>(t, brick_form(meta, slot), slot == target ? tword : n.child[0],
                             uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
                             &p, &r))
           goto hit_found;
@@ -297,8 +301,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       // the child word is read only when the occupancy test passes (pure
       // reads: the same decision as testing the child first)
       uint32_t tck = INVALID_INDEX;
-      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull)
-        tck = n.child[target];
+      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
       if (tck != INVALID_INDEX) {
         // PUSH (:484-492)
         bounds = tb;
@@ -342,6 +345,165 @@ hit_found:
   // the normal is derived once, after the loops, from the hit cell and point
   r.hit = 1;
   r.normal = cube_impact_normal(r.cell, r.impact);
+  return r;
+}
+
+// The same traversal as cast_ray_impl<false, P2> with the work of a wave
+// regrouped (the "while-while" arrangement of GPU ray traversal): each lane
+// runs stack-loop iterations until its next iteration starts with a brick
+// probe, or its ray ends; the wave then probes the due bricks of all such
+// lanes together, and the rounds repeat. Every lane performs the same
+// operations in the same order as in cast_ray_impl (an iteration interrupted
+// for its probe resumes after it, re-reading its node record and without
+// counting a second step), so results are identical bit for bit; only the
+// interleaving across lanes differs, which keeps more lanes active in the
+// brick DDA loop.
+template <bool P2, class LutT>
+SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
+    const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
+    uint32_t ntop) {
+  CastResult r;
+  r.hit = 0;
+  r.pix = EMPTY_MARKER;
+  r.flags = 0;
+  r.impact = V3f{0.0f, 0.0f, 0.0f};
+  r.normal = V3f{0.0f, 0.0f, 0.0f};
+  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
+
+  const V3f scale = dda_scale_factors(d);
+  const uint32_t dir_idx = hash_direction(d);
+  const V3f sig{rsignum(d.x), rsignum(d.y), rsignum(d.z)};
+  const Cube root{v3u(0.0f), t.size};
+  const float lo = 0.00001f;
+  const float hi = 4.0f - 0.00001f;
+
+  uint32_t stk[4] = {0u, 0u, 0u, 0u};
+  uint32_t head = 1u, count = 1u;
+  Cube bounds = root;
+  V3f p;
+  uint32_t target;
+  {
+    float tt = 0.0f;
+    const int ir = intersect_ray(root, o, d, &tt);
+    if (ir != 0) {
+      p = o + d * (ir == 2 ? tt : 0.0f);
+      target = hash_region(p - root.min, root.size / 2.0f);
+    } else {
+      p = o;
+      target = OOB_OCTANT;
+    }
+  }
+  bool finished = target == OOB_OCTANT;
+  bool probed = false;  // the current iteration's probe has run
+  uint32_t steps = 0;
+  for (;;) {
+    // phase 1: stack-loop iterations up to the next due probe
+    while (!finished) {
+      if (count == 0u) {
+        // restart from the root (:551-562)
+        p = p + d * 0.1f;
+        target = (p.x < t.size && p.y < t.size && p.z < t.size && p.x > 0.0f && p.y > 0.0f &&
+                  p.z > 0.0f)
+                     ? hash_region(p, t.size / 2.0f)
+                     : OOB_OCTANT;
+        if (target == OOB_OCTANT) {
+          finished = true;
+          break;
+        }
+        bounds = root;
+        head = (head + 1u) & 3u;
+        count = 1u;
+        stk[head] = 0u;
+      }
+      if (!probed && ++steps > t.max_steps) {
+        r.flags |= FLAG_ITERATION_GUARD;
+        finished = true;
+        break;
+      }
+      const uint32_t key = stk[head];
+      const NodeRec& n = (key < ntop ? top : t.nodes)[key];
+      const uint64_t occ = n.occ;
+      const uint32_t meta = n.meta;
+      const uint32_t tword = n.child[target & 7u];
+      const uint32_t kind = node_kind(meta);
+      if (!probed && target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF))
+        break;  // probe due
+      probed = false;
+      const bool backtrack = kind == KIND_UNIFORM_LEAF;
+      const float rcp_size = size_rcp<P2>(bounds.size);
+      const V3f rel = (p - bounds.min) * 4.0f;
+      V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
+      bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
+      uint32_t flat = bitmap_index(bp, &r.flags);
+      if (backtrack || target == OOB_OCTANT || occ == 0ull ||
+          (occ & lut[flat * 8u + dir_idx]) == 0ull) {
+        // POP (:445-475)
+        head = (head + 3u) & 3u;
+        count -= 1u;
+        if (count != 0u) {
+          const V3f center = bounds.min + v3u(bounds.size / 2.0f);
+          const float s2 = bounds.size * 2.0f;
+          const float rcp_s2 = size_rcp<P2>(s2);
+          const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
+                         sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
+                         sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
+          const uint32_t from = hash_region(center - pmin, bounds.size);
+          const V3f st = dda_step(d, sig, &p, bounds, scale);
+          target = step_octant(from, st);
+          bounds.size *= 2.0f;
+          bounds.min = pmin;
+        }
+        continue;
+      }
+      Cube tb = child_bounds_for(bounds, target);
+      uint32_t tck = INVALID_INDEX;
+      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
+      if (tck != INVALID_INDEX) {
+        // PUSH (:484-492)
+        bounds = tb;
+        target = hash_region(p - tb.min, tb.size / 2.0f);
+        head = (head + 1u) & 3u;
+        count = count < 4u ? count + 1u : 4u;
+        stk[head] = tck;
+      } else {
+        // ADVANCE (:497-544)
+        for (;;) {
+          if (++steps > t.max_steps) {
+            r.flags |= FLAG_ITERATION_GUARD;
+            finished = true;
+            break;
+          }
+          const V3f st = dda_step(d, sig, &p, tb, scale);
+          target = step_octant(target, st);
+          if (target == OOB_OCTANT) break;
+          tb = child_bounds_for(bounds, target);
+          const V3f sd = st * 4.0f;
+          bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
+          flat = bitmap_index(bp, &r.flags);
+          if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
+          if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
+              (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
+            break;
+        }
+      }
+    }
+    if (finished) break;
+    // phase 2: the due probe of the iteration, lanes of the wave together
+    {
+      const NodeRec& n = (stk[head] < ntop ? top : t.nodes)[stk[head]];
+      const uint32_t meta = n.meta;
+      const bool uniform = node_kind(meta) == KIND_UNIFORM_LEAF;
+      const uint32_t slot = uniform ? 0u : target;
+      if (probe_brick<P2>(t, brick_form(meta, slot), n.child[slot],
+                          uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
+                          &p, &r)) {
+        r.hit = 1;
+        r.normal = cube_impact_normal(r.cell, r.impact);
+        return r;
+      }
+      probed = true;
+    }
+  }
   return r;
 }
 

@@ -273,6 +273,8 @@ def test_resumable_traversal_equals_cast_ray():
         base = hostsim.cast(a, rays)
         res = hostsim.cast(a, rays, resumable=True)
         assert base.tobytes() == res.tobytes()
+        # the while-while arrangement (cast_ray_ww) too
+        assert base.tobytes() == hostsim.cast(a, rays, regrouped=True).tobytes()
         exp = oracle.OracleTree(t.export_model()).cast(rays)
         r = compare(exp, {k: res[k] for k in res.dtype.names})
         assert all(v == 0 for k, v in r.items() if k != "rays"), r
@@ -303,6 +305,21 @@ def test_p2_and_general_forms_agree():
             exp = o.cast(rays) if vd is None else o.cast(rays, viewing_distance=vd)
             r = compare(exp, {k: p2[k] for k in p2.dtype.names})
             assert all(v == 0 for k, v in r.items() if k != "rays"), (size, dim, vd, r)
+
+
+def test_regrouped_traversal_under_forced_guard():
+    """cast_ray_ww counts iterations exactly as cast_ray (an iteration
+    resumed after its probe is not counted twice): with lowered bounds both
+    end the same rays."""
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=1)
+    vp = W.viewports(cfg)[0]
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width,
+                                cfg.height)[::5]
+    a = tree.flatten().arrays()
+    for limit in (1, 2, 3, 5, 8, 13):
+        base = hostsim.cast(a, rays, max_steps=limit)
+        assert base.tobytes() == hostsim.cast(a, rays, max_steps=limit, regrouped=True).tobytes()
 
 
 def test_forced_iteration_guard_reports_and_ends_rays():
