@@ -494,9 +494,11 @@ int svo_tree_load(const char* path, svo_octree** host_out, svo_tree** out);
 
 /* Launch variants of the render kernel (no counterpart in the reference,
  * whose WGSL dispatch is fixed at 8x8 workgroups): occupancy hint, placement
- * of the occupancy LUT (global / LDS) and per-wave tile shape. The selection
- * is process-wide and applies to later svo_render calls; every variant
- * computes identical results. */
+ * of the occupancy LUT (global / LDS), per-wave tile shape, top node records
+ * in LDS, workgroup order and the arrangement of the traversal loop. The
+ * selection is process-wide and applies to later svo_render calls; every
+ * variant computes identical results. The initial selection is
+ * "w6-slut-8x8-ww", the fastest on the 1080p caves workload. */
 int svo_render_variant_count(void);
 const char* svo_render_variant_name(int variant);
 int svo_select_render_variant(int variant);

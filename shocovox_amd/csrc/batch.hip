@@ -140,6 +140,7 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
         d = V3f{rr[3], rr[4], rr[5]};
       }
       if (valid) {
+        // (the while-while form, cast_ray_ww, measured 7-13% slower here)
         const CastResult r = cast_ray<true>(tv, o, d);
         store_result(tv, job.out, idx, r, o);
       }

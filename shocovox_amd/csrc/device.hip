@@ -27,7 +27,8 @@ __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __re
   const float* r = rays + 6ull * i;
   const V3f o{r[0], r[1], r[2]};
   const V3f d{r[3], r[4], r[5]};
-  const CastResult res = cast_ray<true>(t, o, d);
+  const CastResult res = cast_ray_ww<true>(t, t.ray_lut, o, d, static_cast<const NodeRec*>(nullptr),
+                                           0u);
   store_result(t, out, i, res, o);
 }
 
@@ -105,8 +106,11 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
   const uint32_t y = (by * 2u + (wave >> 1)) * TH + (lane / TW);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
-  const CastResult res = WW ? cast_ray_ww<true>(t, lut, cam.origin, d, s_nodes, ntop)
-                            : cast_ray<true>(t, lut, cam.origin, d, s_nodes, ntop);
+  CastResult res;
+  if constexpr (WW)
+    res = cast_ray_ww<true>(t, lut, cam.origin, d, s_nodes, ntop);
+  else
+    res = cast_ray<true>(t, lut, cam.origin, d, s_nodes, ntop);
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
@@ -178,7 +182,8 @@ __global__ __launch_bounds__(256) void shade_kernel(TreeView t, ShadeParams sp, 
     o = glass;
     d = normalized(glass - sp.origin);
   }
-  const CastResult r = cast_ray<true>(t, o, d);
+  const CastResult r =
+      cast_ray_ww<true>(t, t.ray_lut, o, d, static_cast<const NodeRec*>(nullptr), 0u);
   uint32_t rgba;
   if (r.hit) {
     const uint32_t ci = pix_color_index(r.pix);
@@ -242,7 +247,12 @@ const RenderVariant kRenderVariants[] = {
     {"r1-t4-f16", render_refill_kernel<1, 4, 16>, 8, 8, 4},
 };
 constexpr int kRenderVariantCount = sizeof(kRenderVariants) / sizeof(kRenderVariants[0]);
-int g_render_variant = 0;
+int initial_render_variant() {
+  for (int i = 0; i < kRenderVariantCount; ++i)
+    if (std::strcmp(kRenderVariants[i].name, "w6-slut-8x8-ww") == 0) return i;
+  return 0;
+}
+int g_render_variant = initial_render_variant();
 
 // Scatter of patched records: src[i] -> dst[idx[i]] (16-byte quarters per lane)
 __global__ __launch_bounds__(256) void scatter_records_kernel(NodeRec* __restrict__ dst,

@@ -357,7 +357,8 @@ hit_found:
 // for its probe resumes after it, re-reading its node record and without
 // counting a second step), so results are identical bit for bit; only the
 // interleaving across lanes differs, which keeps more lanes active in the
-// brick DDA loop.
+// brick DDA loop. (Ending a round's first phase early, once a quorum of
+// the wave's lanes wait for their probes, measured slower.)
 template <bool P2, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
@@ -398,6 +399,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
   uint32_t steps = 0;
   for (;;) {
     // phase 1: stack-loop iterations up to the next due probe
+    bool due = false;
     while (!finished) {
       if (count == 0u) {
         // restart from the root (:551-562)
@@ -426,8 +428,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       const uint32_t meta = n.meta;
       const uint32_t tword = n.child[target & 7u];
       const uint32_t kind = node_kind(meta);
-      if (!probed && target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF))
-        break;  // probe due
+      if (!probed && target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
+        due = true;
+        break;
+      }
       probed = false;
       const bool backtrack = kind == KIND_UNIFORM_LEAF;
       const float rcp_size = size_rcp<P2>(bounds.size);
@@ -488,6 +492,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       }
     }
     if (finished) break;
+    if (!due) continue;
     // phase 2: the due probe of the iteration, lanes of the wave together
     {
       const NodeRec& n = (stk[head] < ntop ? top : t.nodes)[stk[head]];
