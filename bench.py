@@ -8,10 +8,18 @@ renders all 8 poses on the device (rays generated in the kernel, 24 bytes of
 output per ray: hit, voxel xyz, albedo, distance); the tree is resident in
 HBM before timing starts. Prints one JSON line.
 
-The roofline object prices the traversal kernel against HBM bandwidth with
-the algorithmic bytes per ray counted by the instrumented CPU restatement on
-a sample of the same rays; the cpu_baseline object times that restatement
-on the host cores.
+By default a step is one batched launch (svo_batch_run: the 8 render jobs in
+one persistent kernel), whose outputs are checked equal to 8 separate
+svo_render launches; `--launch per-frame` times the 8 separate launches
+instead. A single launch ends when its slowest waves end (a few tiles take
+~10x the mean number of traversal steps), so one launch per frame leaves the
+GPU partly idle at the end of every frame; the batched launch overlaps those
+tails with other frames' tiles. Both figures are reported.
+
+The roofline object prices the timed launch against HBM bandwidth with the
+algorithmic bytes per ray counted by the instrumented CPU restatement on a
+sample of the same rays; the cpu_baseline object times that restatement on
+the host cores.
 """
 from __future__ import annotations
 
@@ -37,9 +45,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, tree, vps, seconds: float) -> tuple[dict, dict]:
+def cpu_baseline(cfg, tree, vps, seconds: float, extra_rays=None) -> tuple[dict, dict]:
     """Times the CPU restatement (single thread, then all cores) on a bounded
-    sample of the same rays and returns (cpu_baseline, per-ray event counts)."""
+    sample of the same rays and returns (cpu_baseline, per-ray event counts of
+    the step's rays: the primary rays plus `extra_rays` if given)."""
     import numpy as np
 
     import oracle
@@ -51,8 +60,12 @@ def cpu_baseline(cfg, tree, vps, seconds: float) -> tuple[dict, dict]:
                                                 cfg.width, cfg.height) for v in vps])
     # event counts on a strided sample (instrumented run)
     _, cnt = otree.cast(rays[::97], counters=True)
-    n = cnt.rays
-    per_ray = {k: getattr(cnt, k) / n for k in EVENT_BYTES}
+    per_ray = {k: getattr(cnt, k) / cnt.rays for k in EVENT_BYTES}
+    if extra_rays is not None:
+        _, ce = otree.cast(extra_rays[::97], counters=True)
+        wp, we = len(rays), len(extra_rays)
+        per_ray = {k: (per_ray[k] * wp + getattr(ce, k) / ce.rays * we) / (wp + we)
+                   for k in EVENT_BYTES}
     # single thread: a strided sample sized to about `seconds / 2`
     probe = rays[::4001]
     t0 = time.perf_counter()
@@ -311,6 +324,11 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--launch", default="batched", choices=["batched", "per-frame"],
+                    help="one batched launch per step (default) or one render launch per pose")
+    ap.add_argument("--batch-variant", default="auto",
+                    help="batch kernel variant of the batched step (name or index; default: "
+                         "autotune)")
     ap.add_argument("--variant", default="auto",
                     help="render launch variant name or index; 'auto' times each and keeps "
                          "the fastest before the timed region")
@@ -344,8 +362,9 @@ def main() -> int:
     if args.mode == "copy":
         return copy_calibration(args)
 
-    from shocovox_amd import (DeviceTree, alloc_hits, collect_kernel_ms, render_variants,
-                              select_render_variant, set_timing, workloads as W)
+    from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variants, collect_kernel_ms,
+                              render_variants, select_batch_variant, select_render_variant,
+                              set_timing, workloads as W)
 
     cfg = W.CONFIGS[args.workload]
     t0 = time.time()
@@ -361,31 +380,31 @@ def main() -> int:
     stream = torch.cuda.current_stream()
 
     incoherent = None
+    inc_host = None
     if cfg.name == "spheres":
         # the 2048^3 workload adds 8M incoherent rays (origins in the volume,
         # directions uniform on the sphere) to every step
-        inc = W.incoherent_rays(cfg.size, 8_000_000, seed=args.seed + 6)
-        incoherent = (torch.from_numpy(inc).to(dev), alloc_hits(len(inc), dev, fields))
-        rays_per_step += len(inc)
-        del inc
+        inc_host = W.incoherent_rays(cfg.size, 8_000_000, seed=args.seed + 6)
+        incoherent = (torch.from_numpy(inc_host).to(dev), alloc_hits(len(inc_host), dev, fields))
+        rays_per_step += len(inc_host)
 
-    def step():
+    def per_frame_step():
         for vp, b in zip(vps, bufs):
             dtree.render_into(vp, cfg.width, cfg.height, b, stream)
         if incoherent is not None:
             dtree.cast_into(incoherent[0], incoherent[0].shape[0], incoherent[1], stream)
 
+    # one launch per pose: autotune the render kernel's launch variants
     names = render_variants()
     tuning = {}
     if args.variant == "auto":
-        # autotune: time every launch variant on the same frames, keep the fastest
         set_timing(True)
         for i, name in enumerate(names):
             select_render_variant(i)
-            step()
+            per_frame_step()
             collect_kernel_ms()
-            step()
-            step()
+            per_frame_step()
+            per_frame_step()
             ms = collect_kernel_ms()
             if incoherent is not None:
                 ms = ms[0::2]  # the render launches; the casts do not depend on the variant
@@ -397,52 +416,121 @@ def main() -> int:
         best = names[int(args.variant)] if args.variant.isdigit() else args.variant
     select_render_variant(names.index(best))
 
+    def time_steps(step, n):
+        """Runs `step` n times between events on the launch stream; returns
+        (wall seconds, average event-timed ms per step)."""
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(n)]
+        t0 = time.perf_counter()
+        for a, b in evs:
+            a.record(stream)
+            step()
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, sum(a.elapsed_time(b) for a, b in evs) / max(1, n)
+
+    for _ in range(args.warmup):
+        per_frame_step()
+    pf_wall, pf_ms = time_steps(per_frame_step, max(3, args.steps // 2))
+    pf_steps = max(3, args.steps // 2)
+    per_frame = {"render_variant": best, "ms_per_step": round(pf_wall / pf_steps * 1e3, 4),
+                 "mrays_per_s": round(rays_per_step * pf_steps / pf_wall / 1e6, 1),
+                 "launches_per_step": len(vps) + (incoherent is not None)}
+    log(f"per-frame launches: {per_frame}")
+    # reference outputs of the step, for the batched launch's equality check
+    per_frame_out = [{k: b[k].clone() for k in fields} for b in bufs]
+    if incoherent is not None:
+        per_frame_out.append({k: incoherent[1][k].clone() for k in fields})
+
+    batch = None
+    batch_tuning = {}
+    bnames = batch_variants()
+    if args.launch == "batched":
+        outs = bufs + ([incoherent[1]] if incoherent is not None else [])
+
+        def make(vi):
+            select_batch_variant(vi)
+            bt = Batch(len(outs))
+            jobs = [Batch.render_job(dtree, vp, cfg.width, cfg.height, b)
+                    for vp, b in zip(vps, bufs)]
+            if incoherent is not None:
+                jobs.append(Batch.cast_job(dtree, incoherent[0], incoherent[1]))
+            return bt, jobs
+
+        if args.batch_variant == "auto":
+            for vi, name in enumerate(bnames):
+                if name.startswith("r1-"):
+                    continue  # lane refill: measured 2x slower on coherent frames
+                bt, jobs = make(vi)
+                bt.run(jobs, stream)
+                _, ms = time_steps(lambda: bt.run(jobs, stream), 3)
+                batch_tuning[name] = round(ms, 4)
+                del bt
+            bvariant = min(batch_tuning, key=batch_tuning.get)
+            log(f"batch autotune (ms per step): {batch_tuning} -> {bvariant}")
+        else:
+            bvariant = (bnames[int(args.batch_variant)] if args.batch_variant.isdigit()
+                        else args.batch_variant)
+        batch, jobs = make(bnames.index(bvariant))
+
+        def step():
+            batch.run(jobs, stream)
+    else:
+        bvariant = None
+        step = per_frame_step
+
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
-    set_timing(True)
-    collect_kernel_ms()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    set_timing(False)
-    kernel_ms = collect_kernel_ms(cap=args.steps * (len(vps) + 1) + 16)
-    if incoherent is not None:
-        kernel_ms = kernel_ms[0::2]  # render launches: the roofline prices the render kernel
+    elapsed, avg_launch_ms = time_steps(step, args.steps)
     ms_per_step = elapsed / args.steps * 1e3
     value = rays_per_step * args.steps / elapsed / 1e6
-    avg_launch_ms = sum(kernel_ms) / max(1, len(kernel_ms))
+    equal = None
+    if batch is not None:
+        outs = bufs + ([incoherent[1]] if incoherent is not None else [])
+        equal = all(torch.equal(o[k], r[k]) for o, r in zip(outs, per_frame_out) for k in fields)
+        if not equal:
+            raise SystemExit("batched launch outputs differ from the per-frame launches")
     hits = float(torch.stack([b["hit"].float().mean() for b in bufs]).mean())
+    launch_desc = (f"one batched launch per step ({bvariant}: {len(vps)} render jobs"
+                   f"{' + 1 cast job' if incoherent is not None else ''})" if batch is not None
+                   else f"{per_frame['launches_per_step']} launches per step ({best})")
     log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} Mrays/s, "
-        f"avg launch {avg_launch_ms:.4f} ms over {len(kernel_ms)} launches, hit rate {hits:.3f}")
+        f"{launch_desc}, event-timed {avg_launch_ms:.4f} ms per step, hit rate {hits:.3f}")
 
     roofline = None
     cpu = None
     if not args.no_cpu_baseline:
-        cpu, per_ray = cpu_baseline(cfg, tree, vps, args.cpu_seconds)
+        cpu, per_ray = cpu_baseline(cfg, tree, vps, args.cpu_seconds, inc_host)
         b_ray = sum(EVENT_BYTES[k] * v for k, v in per_ray.items()) + OUT_BYTES_PER_RAY
-        achieved = b_ray * rays_per_frame / (avg_launch_ms * 1e-3) / 1e9
+        # the timed unit: one batched launch (all rays of the step), or one
+        # per-frame render launch (the render rays of one pose)
+        if batch is not None:
+            rays_per_launch, launch_ms, kname = rays_per_step, avg_launch_ms, "batch_kernel"
+        else:
+            rays_per_launch = rays_per_frame
+            launch_ms = tuning.get(best) if tuning else pf_ms / len(vps)
+            kname = "render_kernel"
+        achieved = b_ray * rays_per_launch / (launch_ms * 1e-3) / 1e9
         traffic = None
         counters = {}
         # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
         # workload, scaled by the same counters' reading of a copy of known size
-        tf = ROOT / "profiles" / f"render_kernel_{cfg.name}_{cfg.height}p_traffic.json"
+        tf = ROOT / "profiles" / f"{kname}_{cfg.name}_{cfg.height}p_traffic.json"
         if tf.exists():
             tj = json.loads(tf.read_text())
-            if tj.get("units_per_launch") == rays_per_frame:
+            if tj.get("units_per_launch") == rays_per_launch:
                 traffic = tj.get("bytes_per_launch")
                 counters = {k: tj[k] for k in ("l2_hit_rate", "valu_lane_utilization")
                             if k in tj}
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic, "kernel": "render_kernel",
+                    "traffic": traffic, "kernel": kname,
                     "bytes_per_ray": round(b_ray, 1),
                     "events_per_ray": {k: round(v, 3) for k, v in per_ray.items()},
-                    "avg_launch_ms": round(avg_launch_ms, 4),
-                    "rays_per_launch": rays_per_frame, **counters}
+                    "avg_launch_ms": round(launch_ms, 4),
+                    "rays_per_launch": rays_per_launch, **counters}
         log(f"cpu baseline {cpu}")
 
     line = {
@@ -451,19 +539,23 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: {cfg.size}^3 octree, brick dim {cfg.brick_dim}, "
-                               f"{cfg.width}x{cfg.height} primary rays x {len(vps)} poses",
+                               f"{cfg.width}x{cfg.height} primary rays x {len(vps)} poses"
+                               + (", + 8M incoherent rays" if incoherent is not None else ""),
                    "octree_size": cfg.size, "brick_dim": cfg.brick_dim, "width": cfg.width,
                    "height": cfg.height, "poses": len(vps), "rays_per_step": rays_per_step,
                    "seed": args.seed, "hit_rate": round(hits, 4),
-                   "device_bytes": dtree.device_bytes, "launch_variant": best,
-                   "autotune_ms": tuning or None},
+                   "device_bytes": dtree.device_bytes, "launch": launch_desc,
+                   "batch_variant": bvariant, "batch_autotune_ms": batch_tuning or None,
+                   "outputs_equal_to_per_frame": equal, "per_frame": per_frame,
+                   "launch_variant": best, "autotune_ms": tuning or None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if not args.no_batched:
         line["batched"] = batched_bench(args, best)
     if not args.no_served:
-        line["served"] = served_bench(args, value)
+        # offered loads as fractions of the one-launch-per-frame capacity
+        line["served"] = served_bench(args, per_frame["mrays_per_s"])
     print(json.dumps(line), flush=True)
     return 0
 

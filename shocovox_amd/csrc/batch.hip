@@ -10,14 +10,18 @@
 // chunks; every wave leaves when all chunks are empty.
 //
 // Kernel variants: "classic" takes one tile per grab and runs it as one ray
-// per lane (with an occupancy hint of 1 or 5 waves per SIMD); the refill
+// per lane (with an occupancy hint of 1 or 5 waves per SIMD), "ww" likewise
+// with the while-while form of the traversal (cast_ray_ww); the refill
 // variants take TPW tiles per grab and run them with lane refill
 // (refill.hpp), one job segment at a time.
 //
 // The per-run sequence - job table upload from pinned memory (only the bytes
 // the run uses), counter reset (a fill kernel, not a memset), persistent
-// kernel - is a hipGraph built once per context; each run sets the copy size
-// and replays it.
+// kernel - is a hipGraph built once per context; each run sets the copy's
+// source and size and replays it. Two pinned stagings alternate, so the host
+// fills the next run's table while the previous run executes; a run waits on
+// its stream for the previous run (which may have been issued on another
+// stream) before its copy overwrites the device table.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -93,7 +97,7 @@ __device__ inline const TreeView* trees_of(const char* table) {
   return reinterpret_cast<const TreeView*>(table + header_of(table)->trees_offset);
 }
 
-template <int MIN_WAVES>
+template <int MIN_WAVES, bool WW = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters) {
   const BatchHeader* hdr = header_of(table);
@@ -140,8 +144,11 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
         d = V3f{rr[3], rr[4], rr[5]};
       }
       if (valid) {
-        // (the while-while form, cast_ray_ww, measured 7-13% slower here)
-        const CastResult r = cast_ray<true>(tv, o, d);
+        CastResult r;
+        if constexpr (WW)
+          r = cast_ray_ww<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(nullptr), 0u);
+        else
+          r = cast_ray<true>(tv, o, d);
         store_result(tv, job.out, idx, r, o);
       }
     }
@@ -197,13 +204,17 @@ using BatchFn = void (*)(const char*, Counter*);
 struct BatchVariant {
   const char* name;
   BatchFn fn;
+  uint32_t groups_per_cu;  // 256-lane workgroups per CU in the persistent grid
 };
 
 const BatchVariant kBatchVariants[] = {
-    {"classic-w1", batch_kernel<1>},
-    {"classic-w5", batch_kernel<5>},
-    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>},
-    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>},
+    {"classic-w1", batch_kernel<1>, 4},
+    {"classic-w5", batch_kernel<5>, 4},
+    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>, 4},
+    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>, 4},
+    {"classic-w5-g5", batch_kernel<5>, 5},
+    {"ww-w5", batch_kernel<5, true>, 4},
+    {"ww-w6-g6", batch_kernel<6, true>, 6},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int g_batch_variant = 0;
@@ -224,15 +235,18 @@ struct BatchContext {
   int variant = 0;
   uint32_t max_jobs = 0;
   size_t table_bytes = 0;  // capacity: header + jobs + trees
-  void* host = nullptr;    // pinned staging of the table
+  // two pinned stagings of the table, used alternately: a run fills one
+  // while the previous run's copy may still be reading the other
+  void* host[2] = {nullptr, nullptr};
+  uint32_t next = 0;       // staging of the next run
   void* dev = nullptr;     // device table
   Counter* counters = nullptr;
   uint32_t grid = 0;
   hipGraph_t graph = nullptr;
   hipGraphNode_t copy_node = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipEvent_t done = nullptr;
-  bool pending = false;
+  hipEvent_t done[2] = {nullptr, nullptr};  // end of the last run that read host[i]
+  bool pending[2] = {false, false};
   uint64_t last_tiles = 0;
   // kernel arguments of the graph's kernel nodes
   const char* arg_table = nullptr;
@@ -244,7 +258,7 @@ namespace {
 int build_graph(BatchContext* b, std::string* err) {
   hipError_t e = hipGraphCreate(&b->graph, 0);
   if (e != hipSuccess) return hip_fail(e, "hipGraphCreate", err);
-  e = hipGraphAddMemcpyNode1D(&b->copy_node, b->graph, nullptr, 0, b->dev, b->host,
+  e = hipGraphAddMemcpyNode1D(&b->copy_node, b->graph, nullptr, 0, b->dev, b->host[0],
                               b->table_bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipGraphAddMemcpyNode1D", err);
   b->arg_table = static_cast<const char*>(b->dev);
@@ -302,19 +316,22 @@ int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
   int cus = 0;
   if (e == hipSuccess)
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-  if (e == hipSuccess) e = hipHostMalloc(&b->host, b->table_bytes, hipHostMallocDefault);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i)
+    e = hipHostMalloc(&b->host[i], b->table_bytes, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&b->dev, b->table_bytes);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void**>(&b->counters), kChunks * sizeof(Counter));
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i)
+    e = hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming);
   if (e != hipSuccess) {
     const int st = hip_fail(e, "batch_create", err);
     batch_destroy(b);
     return st;
   }
-  std::memset(b->host, 0, b->table_bytes);
-  // four 256-lane workgroups per CU keep every SIMD at the kernels' occupancy
-  b->grid = static_cast<uint32_t>(std::max(cus, 1)) * 4u;
+  std::memset(b->host[0], 0, b->table_bytes);
+  std::memset(b->host[1], 0, b->table_bytes);
+  // groups_per_cu 256-lane workgroups per CU: that many waves on every SIMD
+  b->grid = static_cast<uint32_t>(std::max(cus, 1)) * kBatchVariants[b->variant].groups_per_cu;
   const int st = build_graph(b, err);
   if (st != SVO_OK) {
     batch_destroy(b);
@@ -327,13 +344,16 @@ int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
 void batch_destroy(BatchContext* b) {
   if (!b) return;
   if (b->device >= 0) (void)hipSetDevice(b->device);
-  if (b->pending) (void)hipEventSynchronize(b->done);
+  for (int i = 0; i < 2; ++i)
+    if (b->pending[i]) (void)hipEventSynchronize(b->done[i]);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->done) (void)hipEventDestroy(b->done);
+  for (int i = 0; i < 2; ++i)
+    if (b->done[i]) (void)hipEventDestroy(b->done[i]);
   if (b->counters) (void)hipFree(b->counters);
   if (b->dev) (void)hipFree(b->dev);
-  if (b->host) (void)hipHostFree(b->host);
+  for (int i = 0; i < 2; ++i)
+    if (b->host[i]) (void)hipHostFree(b->host[i]);
   delete b;
 }
 
@@ -345,13 +365,15 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     *err = "more jobs than the batch was created for";
     return SVO_INVALID_ARGUMENT;
   }
-  // the previous run's graph reads the pinned table: let it finish first
-  if (b->pending) {
-    const hipError_t e = hipEventSynchronize(b->done);
+  // the run two back copied from this staging: let it finish first (the
+  // previous run, on the other staging, may still be in flight)
+  const uint32_t slot = b->next;
+  if (b->pending[slot]) {
+    const hipError_t e = hipEventSynchronize(b->done[slot]);
     if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize", err);
-    b->pending = false;
+    b->pending[slot] = false;
   }
-  char* base = static_cast<char*>(b->host);
+  char* base = static_cast<char*>(b->host[slot]);
   BatchHeader* hdr = reinterpret_cast<BatchHeader*>(base);
   BatchJob* out = reinterpret_cast<BatchJob*>(base + kJobsOffset);
   const size_t trees_offset = trees_offset_for(n);
@@ -415,15 +437,20 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
   hdr->trees_offset = static_cast<uint32_t>(trees_offset);
   b->last_tiles = tiles;
   const size_t used = trees_offset + tree_index.size() * sizeof(TreeView);
-  hipError_t e = hipGraphExecMemcpyNodeSetParams1D(b->exec, b->copy_node, b->dev, b->host, used,
+  hipError_t e = hipGraphExecMemcpyNodeSetParams1D(b->exec, b->copy_node, b->dev, base, used,
                                                    hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipGraphExecMemcpyNodeSetParams1D", err);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (b->pending[slot ^ 1u]) {
+    e = hipStreamWaitEvent(s, b->done[slot ^ 1u], 0);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent", err);
+  }
   e = hipGraphLaunch(b->exec, s);
   if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch", err);
-  e = hipEventRecord(b->done, s);
+  e = hipEventRecord(b->done[slot], s);
   if (e != hipSuccess) return hip_fail(e, "hipEventRecord", err);
-  b->pending = true;
+  b->pending[slot] = true;
+  b->next = slot ^ 1u;
   return SVO_OK;
 }
 

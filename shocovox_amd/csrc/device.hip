@@ -61,8 +61,10 @@ __global__ __launch_bounds__(256) void render_lod_kernel(TreeView t, Camera cam,
   store_result(t, out, y * w + x, res, cam.origin);
 }
 
-// Primary-ray render. A workgroup is four 64-lane waves arranged 2x2; each
-// wave covers a TW x TH pixel tile (TW * TH = 64). MIN_WAVES asks the
+// Primary-ray render. A workgroup is four 64-lane waves arranged 2x2 (G1:
+// a single wave); each wave covers a TW x TH pixel tile (TW * TH = 64). A
+// workgroup's slot is released when its last wave ends, so with four waves
+// the fastest three wait for the slowest. MIN_WAVES asks the
 // compiler for at least that many waves per SIMD (it caps VGPRs to fit);
 // LDS_LUT stages the 4 KiB RAY_TO_NODE_OCCUPANCY_BITMASK_LUT in LDS.
 // NODES > 0 stages the first NODES node records (the top levels of the
@@ -72,15 +74,16 @@ __global__ __launch_bounds__(256) void render_lod_kernel(TreeView t, Camera cam,
 // and bricks in its own L2. WW runs the while-while arrangement of the
 // traversal (cast_ray_ww), which probes the due bricks of a wave together.
 template <int MIN_WAVES, bool LDS_LUT, int TW, int TH, int NODES = 0, bool XCD = false,
-          bool WW = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
+          bool WW = false, bool G1 = false>
+__global__ __launch_bounds__(G1 ? 64 : 256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) {
   static_assert(TW * TH == 64, "a wave covers 64 pixels");
+  constexpr uint32_t kThreads = G1 ? 64u : 256u;
   __shared__ uint64_t s_lut[LDS_LUT ? 512 : 1];
   __shared__ NodeRec s_nodes[NODES > 0 ? NODES : 1];
   const uint64_t* lut = t.ray_lut;
   if constexpr (LDS_LUT) {
-    for (uint32_t i = threadIdx.x; i < 512; i += 256) s_lut[i] = t.ray_lut[i];
+    for (uint32_t i = threadIdx.x; i < 512; i += kThreads) s_lut[i] = t.ray_lut[i];
     lut = s_lut;
   }
   uint32_t ntop = 0;
@@ -88,7 +91,7 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
     ntop = min(static_cast<uint32_t>(NODES), t.node_count);
     const uint4* src = reinterpret_cast<const uint4*>(t.nodes);
     uint4* dst = reinterpret_cast<uint4*>(s_nodes);
-    for (uint32_t i = threadIdx.x; i < ntop * 4u; i += 256) dst[i] = src[i];
+    for (uint32_t i = threadIdx.x; i < ntop * 4u; i += kThreads) dst[i] = src[i];
   }
   if constexpr (LDS_LUT || NODES > 0) __syncthreads();
   uint32_t bx = blockIdx.x, by = blockIdx.y;
@@ -102,8 +105,9 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
   }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t x = (bx * 2u + (wave & 1u)) * TW + (lane % TW);
-  const uint32_t y = (by * 2u + (wave >> 1)) * TH + (lane / TW);
+  const uint32_t gx = G1 ? 1u : 2u;
+  const uint32_t x = (bx * gx + (wave & 1u)) * TW + (lane % TW);
+  const uint32_t y = (by * gx + (wave >> 1)) * TH + (lane / TW);
   if (x >= w || y >= h) return;
   const V3f d = camera_ray_dir(cam, x, y);
   CastResult res;
@@ -214,6 +218,7 @@ struct RenderVariant {
   RenderFn fn;
   uint32_t tile_w, tile_h;
   uint32_t tiles_per_wave;  // refill kernels: 8x8 tiles per wave (0: one tile per wave)
+  uint32_t single_wave_groups;  // 1: one-wave workgroups (G1), 0: 2x2 waves
 };
 
 const RenderVariant kRenderVariants[] = {
@@ -241,6 +246,10 @@ const RenderVariant kRenderVariants[] = {
     {"w5-glut-8x8-ww", render_kernel<5, false, 8, 8, 0, false, true>, 8, 8},
     {"w6-slut-8x8-ww", render_kernel<6, true, 8, 8, 0, false, true>, 8, 8},
     {"w5-slut-8x8-n73-ww", render_kernel<5, true, 8, 8, 73, false, true>, 8, 8},
+    {"w6-glut-8x8-ww-g1", render_kernel<6, false, 8, 8, 0, false, true, true>, 8, 8, 0, 1},
+    {"w6-slut-8x8-ww-g1", render_kernel<6, true, 8, 8, 0, false, true, true>, 8, 8, 0, 1},
+    {"w5-glut-8x8-ww-g1", render_kernel<5, false, 8, 8, 0, false, true, true>, 8, 8, 0, 1},
+    {"w6-glut-8x8-g1", render_kernel<6, false, 8, 8, 0, false, false, true>, 8, 8, 0, 1},
     // lane refill measured 3.5x slower than one tile per wave on coherent
     // 1080p primary rays (lanes drift apart and lose the shared node lines);
     // kept as a variant for divergent workloads
@@ -790,7 +799,8 @@ int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
   const RenderVariant& v = kRenderVariants[g_render_variant];
-  dim3 grid((w + 2 * v.tile_w - 1) / (2 * v.tile_w), (h + 2 * v.tile_h - 1) / (2 * v.tile_h));
+  const uint32_t gx = v.single_wave_groups ? 1u : 2u;
+  dim3 grid((w + gx * v.tile_w - 1) / (gx * v.tile_w), (h + gx * v.tile_h - 1) / (gx * v.tile_h));
   if (v.tiles_per_wave) {
     // four waves per workgroup, each over tiles_per_wave consecutive 8x8 tiles
     const uint64_t tiles = static_cast<uint64_t>((w + 7) / 8) * ((h + 7) / 8);
@@ -801,7 +811,7 @@ int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h
     const uint64_t waves = (tiles + v.tiles_per_wave - 1) / v.tiles_per_wave;
     grid = dim3(static_cast<uint32_t>((waves + 3) / 4));
   }
-  hipLaunchKernelGGL(v.fn, grid, dim3(256), 0, s, t.view, cam, w, h, to_ptrs(out));
+  hipLaunchKernelGGL(v.fn, grid, dim3(v.single_wave_groups ? 64 : 256), 0, s, t.view, cam, w, h, to_ptrs(out));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "render_kernel launch", err);
   timer.done();
