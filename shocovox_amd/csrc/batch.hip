@@ -215,6 +215,7 @@ const BatchVariant kBatchVariants[] = {
     {"classic-w5-g5", batch_kernel<5>, 5},
     {"ww-w5", batch_kernel<5, true>, 4},
     {"ww-w6-g6", batch_kernel<6, true>, 6},
+    {"ww-w5-g5", batch_kernel<5, true>, 5},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int g_batch_variant = 0;

@@ -129,6 +129,18 @@ SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
   return x + 4u * y + 16u * z;
 }
 
+// A finished ray that hit nothing, as cast_ray returns it.
+SVO_HD inline CastResult miss_result(uint32_t flags) {
+  CastResult r;
+  r.hit = 0;
+  r.pix = EMPTY_MARKER;
+  r.flags = flags;
+  r.impact = V3f{0.0f, 0.0f, 0.0f};
+  r.normal = V3f{0.0f, 0.0f, 0.0f};
+  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
+  return r;
+}
+
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
 template <bool P2>
 SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t, uint32_t form,
@@ -200,13 +212,12 @@ template <bool LOD, bool P2, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop, float viewing_distance) {
+  // Only the flags live across the loops; the probe fills pix, impact and
+  // cell on the hit path, and a miss returns miss_result(flags). (Zeroing
+  // the hit fields up front would make the compiler carry them around every
+  // loop.)
   CastResult r;
-  r.hit = 0;
-  r.pix = EMPTY_MARKER;
   r.flags = 0;
-  r.impact = V3f{0.0f, 0.0f, 0.0f};
-  r.normal = V3f{0.0f, 0.0f, 0.0f};
-  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
 
   const V3f scale = dda_scale_factors(d);
   const uint32_t dir_idx = hash_direction(d);
@@ -241,10 +252,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     count = count < 4u ? count + 1u : 4u;
     stk[head] = 0u;
     while (count != 0u) {
-      if (++steps > t.max_steps) {
-        r.flags |= FLAG_ITERATION_GUARD;
-        return r;
-      }
+      if (++steps > t.max_steps) return miss_result(r.flags | FLAG_ITERATION_GUARD);
       const uint32_t key = stk[head];
       const NodeRec& n = (key < ntop ? top : t.nodes)[key];
       const uint64_t occ = n.occ;
@@ -313,10 +321,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       } else {
         // ADVANCE (:497-544)
         for (;;) {
-          if (++steps > t.max_steps) {
-            r.flags |= FLAG_ITERATION_GUARD;
-            return r;
-          }
+          if (++steps > t.max_steps) return miss_result(r.flags | FLAG_ITERATION_GUARD);
           const V3f st = dda_step(d, sig, &p, tb, scale);
           target = step_octant(target, st);
           if (target == OOB_OCTANT) break;
@@ -340,7 +345,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
                  ? hash_region(p, t.size / 2.0f)
                  : OOB_OCTANT;
   }
-  return r;
+  return miss_result(r.flags);
 hit_found:
   // the normal is derived once, after the loops, from the hit cell and point
   r.hit = 1;
@@ -363,13 +368,9 @@ template <bool P2, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop) {
-  CastResult r;
-  r.hit = 0;
-  r.pix = EMPTY_MARKER;
-  r.flags = 0;
-  r.impact = V3f{0.0f, 0.0f, 0.0f};
-  r.normal = V3f{0.0f, 0.0f, 0.0f};
-  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
+  // (here, unlike in cast_ray_impl, zeroing the hit fields up front gives
+  // the smaller register allocation)
+  CastResult r = miss_result(0u);
 
   const V3f scale = dda_scale_factors(d);
   const uint32_t dir_idx = hash_direction(d);
@@ -556,18 +557,6 @@ struct RayState {
     stk3 = i == 3u ? v : stk3;
   }
 };
-
-// A finished ray that hit nothing, as cast_ray returns it.
-SVO_HD inline CastResult miss_result(uint32_t flags) {
-  CastResult r;
-  r.hit = 0;
-  r.pix = EMPTY_MARKER;
-  r.flags = flags;
-  r.impact = V3f{0.0f, 0.0f, 0.0f};
-  r.normal = V3f{0.0f, 0.0f, 0.0f};
-  r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
-  return r;
-}
 
 // false when the ray misses the tree outright (its result is miss_result(0))
 SVO_HD __attribute__((always_inline)) inline bool ray_begin(const TreeView& t, V3f o, V3f d,

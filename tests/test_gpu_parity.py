@@ -268,7 +268,7 @@ def test_patch_that_outgrows_the_device_arrays(gpu_device):
         assert_parity(_oracle(tree).cast(rays), got, f"growth round {rnd}")
 
 
-@pytest.mark.parametrize("variant", range(7))
+@pytest.mark.parametrize("variant", range(8))
 def test_batched_mixed_jobs_and_replays(gpu_device, variant):
     """Many trees, uneven viewports and ray arrays in one persistent launch;
     the graph is replayed with different job sets (with eager work in
@@ -278,7 +278,7 @@ def test_batched_mixed_jobs_and_replays(gpu_device, variant):
     from shocovox_amd import (Batch, Octree, alloc_hits, batch_variants, current_batch_variant,
                               select_batch_variant)
     names = batch_variants()
-    assert len(names) == 7
+    assert len(names) == 8
     _say(f"batch variant {names[variant]}")
     rng = np.random.default_rng(23)
     trees = [W.build_scene(W.PLAIN, seed=s) for s in (1, 2)]
