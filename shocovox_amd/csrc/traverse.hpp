@@ -227,7 +227,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
   const float hi = 4.0f - 0.00001f;
 
   uint32_t stk[4] = {0u, 0u, 0u, 0u};
-  uint32_t head = 0, count = 0;
+  uint32_t head = 1u, count = 1u;  // the root pushed
   Cube bounds = root;
   V3f p;
   uint32_t target;
@@ -242,112 +242,128 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       target = OOB_OCTANT;
     }
   }
+  if (target == OOB_OCTANT) return miss_result(0u);
   uint32_t steps = 0;
   // log2(size / dim) of two powers of two, exactly
   float mip_level =
       LOD ? static_cast<float>(__builtin_ctz(static_cast<uint32_t>(t.size) / t.dim)) : 0.0f;
-  while (target != OOB_OCTANT) {
-    bounds = root;
-    head = (head + 1u) & 3u;
-    count = count < 4u ? count + 1u : 4u;
-    stk[head] = 0u;
-    while (count != 0u) {
-      if (++steps > t.max_steps) return miss_result(r.flags | FLAG_ITERATION_GUARD);
-      const uint32_t key = stk[head];
-      const NodeRec& n = (key < ntop ? top : t.nodes)[key];
-      const uint64_t occ = n.occ;
-      const uint32_t meta = n.meta;
-      // the child word of the current target, read together with the header
-      // (same record, one round trip); with target OOB_OCTANT it is child[0]
-      // and unused
-      const uint32_t tword = n.child[target & 7u];
-      const uint32_t kind = node_kind(meta);
-      bool backtrack = kind == KIND_UNIFORM_LEAF;
-      if (LOD && t.mips_enabled) {
-        const float grid = mip_level * 2.0f;
-        const V3f snapped{roundf(p.x / grid) * grid, roundf(p.y / grid) * grid,
-                          roundf(p.z / grid) * grid};
-        if (mip_level < length(o - snapped) / viewing_distance &&
-            probe_brick<P2>(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r))
-          goto hit_found;
+  // One loop runs the reference's two: an iteration of its stack loop, and
+  // when the stack has emptied, first its restart from the root (which ends
+  // the ray when the nudged point leaves the tree).
+  for (;;) {
+    if (count == 0u) {
+      // restart from the root (:551-562)
+      p = p + d * 0.1f;
+      target = (p.x < t.size && p.y < t.size && p.z < t.size && p.x > 0.0f && p.y > 0.0f &&
+                p.z > 0.0f)
+                   ? hash_region(p, t.size / 2.0f)
+                   : OOB_OCTANT;
+      if (target == OOB_OCTANT) break;
+      bounds = root;
+      head = (head + 1u) & 3u;
+      count = 1u;
+      stk[head] = 0u;
+    }
+    if (++steps > t.max_steps) {
+      r.flags |= FLAG_ITERATION_GUARD;
+      break;
+    }
+    const uint32_t key = stk[head];
+    const NodeRec& n = (key < ntop ? top : t.nodes)[key];
+    const uint64_t occ = n.occ;
+    const uint32_t meta = n.meta;
+    // the child word of the current target, read together with the header
+    // (same record, one round trip); with target OOB_OCTANT it is child[0]
+    // and unused
+    const uint32_t tword = n.child[target & 7u];
+    const uint32_t kind = node_kind(meta);
+    bool backtrack = kind == KIND_UNIFORM_LEAF;
+    if (LOD && t.mips_enabled) {
+      const float grid = mip_level * 2.0f;
+      const V3f snapped{roundf(p.x / grid) * grid, roundf(p.y / grid) * grid,
+                        roundf(p.z / grid) * grid};
+      if (mip_level < length(o - snapped) / viewing_distance &&
+          probe_brick<P2>(t, n.mip, n.mip_word, bounds, d, sig, scale, &p, &r))
+        goto hit_found;
+    }
+    if (target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
+      const bool uniform = kind == KIND_UNIFORM_LEAF;
+      const uint32_t slot = uniform ? 0u : target;
+      if (probe_brick<P2>(t, brick_form(meta, slot), slot == target ? tword : n.child[0],
+                          uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
+                          &p, &r))
+        goto hit_found;
+    }
+    const float rcp_size = size_rcp<P2>(bounds.size);
+    const V3f rel = (p - bounds.min) * 4.0f;
+    V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
+    bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
+    uint32_t flat = bitmap_index(bp, &r.flags);
+    if (backtrack || target == OOB_OCTANT || occ == 0ull ||
+        (occ & lut[flat * 8u + dir_idx]) == 0ull) {
+      // POP (:445-475)
+      head = (head + 3u) & 3u;
+      count -= 1u;
+      if (LOD) mip_level += 1.0f;
+      if (count != 0u) {
+        const V3f center = bounds.min + v3u(bounds.size / 2.0f);
+        const float s2 = bounds.size * 2.0f;
+        const float rcp_s2 = size_rcp<P2>(s2);
+        const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
+                       sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
+                       sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
+        const uint32_t from = hash_region(center - pmin, bounds.size);
+        const V3f st = dda_step(d, sig, &p, bounds, scale);
+        target = step_octant(from, st);
+        bounds.size *= 2.0f;
+        bounds.min = pmin;
       }
-      if (target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
-        const bool uniform = kind == KIND_UNIFORM_LEAF;
-        const uint32_t slot = uniform ? 0u : target;
-        if (probe_brick<P2>(t, brick_form(meta, slot), slot == target ? tword : n.child[0],
-                            uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
-                            &p, &r))
-          goto hit_found;
-      }
-      const float rcp_size = size_rcp<P2>(bounds.size);
-      const V3f rel = (p - bounds.min) * 4.0f;
-      V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
-      bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
-      uint32_t flat = bitmap_index(bp, &r.flags);
-      if (backtrack || target == OOB_OCTANT || occ == 0ull ||
-          (occ & lut[flat * 8u + dir_idx]) == 0ull) {
-        // POP (:445-475)
-        head = (head + 3u) & 3u;
-        count -= 1u;
-        if (LOD) mip_level += 1.0f;
-        if (count != 0u) {
-          const V3f center = bounds.min + v3u(bounds.size / 2.0f);
-          const float s2 = bounds.size * 2.0f;
-          const float rcp_s2 = size_rcp<P2>(s2);
-          const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
-                         sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
-                         sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
-          const uint32_t from = hash_region(center - pmin, bounds.size);
-          const V3f st = dda_step(d, sig, &p, bounds, scale);
-          target = step_octant(from, st);
-          bounds.size *= 2.0f;
-          bounds.min = pmin;
+      continue;
+    }
+    Cube tb = child_bounds_for(bounds, target);
+    // the child word is read only when the occupancy test passes (pure
+    // reads: the same decision as testing the child first)
+    uint32_t tck = INVALID_INDEX;
+    if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
+    if (tck != INVALID_INDEX) {
+      // PUSH (:484-492)
+      bounds = tb;
+      target = hash_region(p - tb.min, tb.size / 2.0f);
+      head = (head + 1u) & 3u;
+      count = count < 4u ? count + 1u : 4u;
+      stk[head] = tck;
+      if (LOD) mip_level -= 1.0f;
+    } else {
+      // ADVANCE (:497-544)
+      bool guard = false;
+      for (;;) {
+        if (++steps > t.max_steps) {
+          guard = true;
+          break;
         }
-        continue;
+        const V3f st = dda_step(d, sig, &p, tb, scale);
+        target = step_octant(target, st);
+        if (target == OOB_OCTANT) break;
+        tb = child_bounds_for(bounds, target);
+        const V3f sd = st * 4.0f;
+        bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
+        flat = bitmap_index(bp, &r.flags);
+        // stop at a reachable occupied child or a non-empty leaf brick; the
+        // tests run cheapest first and the child word is read last
+        if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
+        if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
+            (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
+          break;
       }
-      Cube tb = child_bounds_for(bounds, target);
-      // the child word is read only when the occupancy test passes (pure
-      // reads: the same decision as testing the child first)
-      uint32_t tck = INVALID_INDEX;
-      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
-      if (tck != INVALID_INDEX) {
-        // PUSH (:484-492)
-        bounds = tb;
-        target = hash_region(p - tb.min, tb.size / 2.0f);
-        head = (head + 1u) & 3u;
-        count = count < 4u ? count + 1u : 4u;
-        stk[head] = tck;
-        if (LOD) mip_level -= 1.0f;
-      } else {
-        // ADVANCE (:497-544)
-        for (;;) {
-          if (++steps > t.max_steps) return miss_result(r.flags | FLAG_ITERATION_GUARD);
-          const V3f st = dda_step(d, sig, &p, tb, scale);
-          target = step_octant(target, st);
-          if (target == OOB_OCTANT) break;
-          tb = child_bounds_for(bounds, target);
-          const V3f sd = st * 4.0f;
-          bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
-          flat = bitmap_index(bp, &r.flags);
-          // stop at a reachable occupied child or a non-empty leaf brick; the
-          // tests run cheapest first and the child word is read last
-          if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
-          if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
-              (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
-            break;
-        }
+      if (guard) {
+        r.flags |= FLAG_ITERATION_GUARD;
+        break;
       }
     }
-    // restart from the root (:551-562)
-    p = p + d * 0.1f;
-    target = (p.x < t.size && p.y < t.size && p.z < t.size && p.x > 0.0f && p.y > 0.0f &&
-              p.z > 0.0f)
-                 ? hash_region(p, t.size / 2.0f)
-                 : OOB_OCTANT;
   }
   return miss_result(r.flags);
 hit_found:
-  // the normal is derived once, after the loops, from the hit cell and point
+  // the normal is derived once, after the loop, from the hit cell and point
   r.hit = 1;
   r.normal = cube_impact_normal(r.cell, r.impact);
   return r;
