@@ -55,12 +55,6 @@ SVO_HD inline float rsignum(float x) {
 }
 
 // `f as i32` (saturating, NaN -> 0)
-// `s as i32` for a DDA step component. dda_step returns, per axis, either
-// +0.0 or signum(direction), i.e. one of {-1, +1, +0, NaN}; on that domain
-// this equals f2i32(s) and f2i32(roundf(s)) (NaN -> 0) without the general
-// conversion's range checks.
-SVO_HD inline int32_t step_int(float s) { return s == 1.0f ? 1 : (s == -1.0f ? -1 : 0); }
-
 SVO_HD inline int32_t f2i32(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   // v_cvt_i32_f32 truncates toward zero, clamps out-of-range values to
@@ -74,6 +68,18 @@ SVO_HD inline int32_t f2i32(float x) {
   if (x >= 2147483648.0f) return 2147483647;
   if (x < -2147483648.0f) return (-2147483647 - 1);
   return static_cast<int32_t>(x);
+}
+
+// `s as i32` for a DDA step component. dda_step returns, per axis, either
+// +0.0 or signum(direction), i.e. one of {-1, +1, +0, NaN}; on that domain
+// this equals f2i32(s) and f2i32(roundf(s)) (NaN -> 0): one conversion
+// instruction on the device, two compares on the host.
+SVO_HD inline int32_t step_int(float s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return f2i32(s);
+#else
+  return s == 1.0f ? 1 : (s == -1.0f ? -1 : 0);
+#endif
 }
 
 // `f as usize` (saturating, NaN -> 0)
@@ -140,12 +146,10 @@ SVO_HD inline uint32_t step_octant_signs(uint32_t octant, int sx, int sy, int sz
   return static_cast<uint32_t>(x + 2 * z + 4 * y);
 }
 
-SVO_HD inline int isignum(int v) { return (v > 0) - (v < 0); }
-
 // step_octant (raytracing/mod.rs:68-80): `(step.c as i32).signum()`
 SVO_HD inline uint32_t step_octant(uint32_t octant, V3f step) {
-  return step_octant_signs(octant, isignum(step_int(step.x)), isignum(step_int(step.y)),
-                           isignum(step_int(step.z)));
+  // step_int is already in {-1, 0, 1}, where signum is the identity
+  return step_octant_signs(octant, step_int(step.x), step_int(step.y), step_int(step.z));
 }
 
 // Cube{min_position, size}  (spatial/mod.rs:31-53)

@@ -170,7 +170,14 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
           unit};
   const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
   for (int guard = 0; guard < 4 * dim + 16; ++guard) {
-    if (ix < 0 || ix >= dim || iy < 0 || iy >= dim || iz < 0 || iz >= dim) return false;
+    if constexpr (P2) {
+      // with dim a power of two, all three are in [0, dim) exactly when
+      // their bitwise or is (a negative index has the top bit set): one
+      // test instead of six
+      if (static_cast<uint32_t>(ix | iy | iz) >= static_cast<uint32_t>(dim)) return false;
+    } else {
+      if (ix < 0 || ix >= dim || iy < 0 || iy >= dim || iz < 0 || iz >= dim) return false;
+    }
     flat = ix + iy * dim + iz * dim * dim;
     if ((occ[static_cast<uint32_t>(flat) >> 5] >> (static_cast<uint32_t>(flat) & 31u)) & 1u) {
       const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
@@ -299,9 +306,35 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
     bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
     uint32_t flat = bitmap_index(bp, &r.flags);
-    if (backtrack || target == OOB_OCTANT || occ == 0ull ||
-        (occ & lut[flat * 8u + dir_idx]) == 0ull) {
-      // POP (:445-475)
+    const bool pop = backtrack || target == OOB_OCTANT || occ == 0ull ||
+                     (occ & lut[flat * 8u + dir_idx]) == 0ull;
+    Cube tb = child_bounds_for(bounds, target);
+    // the child word is read only when the occupancy test passes (pure
+    // reads: the same decision as testing the child first)
+    uint32_t tck = INVALID_INDEX;
+    if (!pop && kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
+    if (tck != INVALID_INDEX) {
+      // PUSH (:484-492)
+      bounds = tb;
+      target = hash_region(p - tb.min, tb.size / 2.0f);
+      head = (head + 1u) & 3u;
+      count = count < 4u ? count + 1u : 4u;
+      stk[head] = tck;
+      if (LOD) mip_level -= 1.0f;
+      continue;
+    }
+    // POP (:445-475) and ADVANCE (:497-544) both begin with one DDA step,
+    // out of the node (POP, unless the stack empties) or out of the target
+    // child (ADVANCE); it is issued once for the lanes of both. (The POP's
+    // other values depend on the bounds only, so taking the step before
+    // them changes nothing.)
+    if (!pop && ++steps > t.max_steps) {
+      r.flags |= FLAG_ITERATION_GUARD;
+      break;
+    }
+    V3f st{0.0f, 0.0f, 0.0f};
+    if (!pop || count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
+    if (pop) {
       head = (head + 3u) & 3u;
       count -= 1u;
       if (LOD) mip_level += 1.0f;
@@ -313,52 +346,36 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
                        sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
                        sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
         const uint32_t from = hash_region(center - pmin, bounds.size);
-        const V3f st = dda_step(d, sig, &p, bounds, scale);
         target = step_octant(from, st);
         bounds.size *= 2.0f;
         bounds.min = pmin;
       }
       continue;
     }
-    Cube tb = child_bounds_for(bounds, target);
-    // the child word is read only when the occupancy test passes (pure
-    // reads: the same decision as testing the child first)
-    uint32_t tck = INVALID_INDEX;
-    if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
-    if (tck != INVALID_INDEX) {
-      // PUSH (:484-492)
-      bounds = tb;
-      target = hash_region(p - tb.min, tb.size / 2.0f);
-      head = (head + 1u) & 3u;
-      count = count < 4u ? count + 1u : 4u;
-      stk[head] = tck;
-      if (LOD) mip_level -= 1.0f;
-    } else {
-      // ADVANCE (:497-544)
-      bool guard = false;
-      for (;;) {
-        if (++steps > t.max_steps) {
-          guard = true;
-          break;
-        }
-        const V3f st = dda_step(d, sig, &p, tb, scale);
-        target = step_octant(target, st);
-        if (target == OOB_OCTANT) break;
-        tb = child_bounds_for(bounds, target);
-        const V3f sd = st * 4.0f;
-        bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
-        flat = bitmap_index(bp, &r.flags);
-        // stop at a reachable occupied child or a non-empty leaf brick; the
-        // tests run cheapest first and the child word is read last
-        if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
-        if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
-            (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
-          break;
-      }
-      if (guard) {
-        r.flags |= FLAG_ITERATION_GUARD;
+    // the rest of ADVANCE
+    bool guard = false;
+    target = step_octant(target, st);
+    while (target != OOB_OCTANT) {
+      tb = child_bounds_for(bounds, target);
+      const V3f sd = st * 4.0f;
+      bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
+      flat = bitmap_index(bp, &r.flags);
+      // stop at a reachable occupied child or a non-empty leaf brick; the
+      // tests run cheapest first and the child word is read last
+      if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
+      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
+          (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
+        break;
+      if (++steps > t.max_steps) {
+        guard = true;
         break;
       }
+      st = dda_step(d, sig, &p, tb, scale);
+      target = step_octant(target, st);
+    }
+    if (guard) {
+      r.flags |= FLAG_ITERATION_GUARD;
+      break;
     }
   }
   return miss_result(r.flags);
