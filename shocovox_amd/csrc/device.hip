@@ -193,7 +193,8 @@ __global__ __launch_bounds__(256) void shade_kernel(TreeView t, ShadeParams sp, 
     const uint32_t ci = pix_color_index(r.pix);
     const uint32_t c = (ci < NONE16 && ci < t.color_count) ? t.colors[ci] : 0u;
     const uint32_t cr = c >> 24, cg = (c >> 16) & 0xFFu, cb = (c >> 8) & 0xFFu;
-    const float dot = (r.normal.x * sp.light.x + r.normal.y * sp.light.y) + r.normal.z * sp.light.z;
+    const V3f nrm = hit_normal(r);
+    const float dot = (nrm.x * sp.light.x + nrm.y * sp.light.y) + nrm.z * sp.light.z;
     if (CONV == 0) {
       const float strength = 1.0f - (dot / 2.0f + 0.5f);
       rgba = sat_u8(static_cast<float>(cr) * strength) |

@@ -49,9 +49,10 @@ __device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint32_
     o.impact[3 * i + 2] = r.impact.z;
   }
   if (o.normal) {
-    o.normal[3 * i + 0] = r.normal.x;
-    o.normal[3 * i + 1] = r.normal.y;
-    o.normal[3 * i + 2] = r.normal.z;
+    const V3f nrm = hit_normal(r);
+    o.normal[3 * i + 0] = nrm.x;
+    o.normal[3 * i + 1] = nrm.y;
+    o.normal[3 * i + 2] = nrm.z;
   }
   if (o.flags) o.flags[i] = r.flags;
 }

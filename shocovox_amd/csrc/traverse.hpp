@@ -96,14 +96,21 @@ SVO_HD inline float sub_fmod_size(float a, float b, float rcp_b) {
   }
 }
 
+// A ray's result. The normal of a hit is a function of the hit cell and the
+// impact point (hit_normal); it is derived only where a caller asks for it.
 struct CastResult {
   uint32_t hit;
   uint32_t pix;
   uint32_t flags;
   V3f impact;
-  V3f normal;
   Cube cell;  // the voxel cell (Parted) or brick (Solid) that was hit
 };
+
+// get_by_ray's normal (raytracing_on_cpu.rs:296-311): cube_impact_normal of
+// the hit cell at the impact point; zero for a miss.
+SVO_HD inline V3f hit_normal(const CastResult& r) {
+  return r.hit ? cube_impact_normal(r.cell, r.impact) : V3f{0.0f, 0.0f, 0.0f};
+}
 
 // Rust f32::clamp(min, max): NaN passes through
 SVO_HD inline float rclamp(float x, float lo, float hi) {
@@ -136,9 +143,15 @@ SVO_HD inline CastResult miss_result(uint32_t flags) {
   r.pix = EMPTY_MARKER;
   r.flags = flags;
   r.impact = V3f{0.0f, 0.0f, 0.0f};
-  r.normal = V3f{0.0f, 0.0f, 0.0f};
   r.cell = Cube{V3f{0.0f, 0.0f, 0.0f}, 0.0f};
   return r;
+}
+
+// bitmap_index of a position clamped to [1e-5, 4 - 1e-5] (or NaN, which the
+// clamp passes through and the conversion maps to 0): every axis is already
+// in 0..3, so the range test is left out.
+SVO_HD inline uint32_t bitmap_index_clamped(V3f bp) {
+  return f2u32(floorf(bp.x)) + 4u * f2u32(floorf(bp.y)) + 16u * f2u32(floorf(bp.z));
 }
 
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
@@ -201,6 +214,30 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   return false;
 }
 
+// The reference's NodeStack<u32, 4> (raytracing_on_cpu.rs:19-82): a ring of
+// four whose push overwrites the oldest entry once full and whose count
+// says how many entries are retained. Kept here as a shift register, s0 the
+// top: a push shifts down (dropping the oldest), a pop shifts up, so the top
+// is read without selecting among the four registers. Entries beyond
+// `count` are never read, which makes the two forms equivalent.
+struct RingStack {
+  uint32_t s0, s1, s2, s3;
+  uint32_t count;
+  SVO_HD void push(uint32_t v) {
+    s3 = s2;
+    s2 = s1;
+    s1 = s0;
+    s0 = v;
+    count = count < 4u ? count + 1u : 4u;
+  }
+  SVO_HD void pop() {
+    s0 = s1;
+    s1 = s2;
+    s2 = s3;
+    count -= 1u;
+  }
+};
+
 // get_by_ray_at_lod (raytracing_on_cpu.rs:325-565). Without LOD it is
 // get_by_ray (viewing distance f32::MAX, under which the MIP test never
 // passes). `lut` is the RAY_TO_NODE_OCCUPANCY_BITMASK_LUT, wherever the
@@ -233,8 +270,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
   const float lo = 0.00001f;
   const float hi = 4.0f - 0.00001f;
 
-  uint32_t stk[4] = {0u, 0u, 0u, 0u};
-  uint32_t head = 1u, count = 1u;  // the root pushed
+  RingStack stk{0u, 0u, 0u, 0u, 1u};  // the root pushed
   Cube bounds = root;
   V3f p;
   uint32_t target;
@@ -258,24 +294,22 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
   // when the stack has emptied, first its restart from the root (which ends
   // the ray when the nudged point leaves the tree).
   for (;;) {
-    if (count == 0u) {
+    if (stk.count == 0u) {
       // restart from the root (:551-562)
       p = p + d * 0.1f;
-      target = (p.x < t.size && p.y < t.size && p.z < t.size && p.x > 0.0f && p.y > 0.0f &&
-                p.z > 0.0f)
+      target = ((p.x < t.size) & (p.y < t.size) & (p.z < t.size) & (p.x > 0.0f) & (p.y > 0.0f) &
+                 (p.z > 0.0f))
                    ? hash_region(p, t.size / 2.0f)
                    : OOB_OCTANT;
       if (target == OOB_OCTANT) break;
       bounds = root;
-      head = (head + 1u) & 3u;
-      count = 1u;
-      stk[head] = 0u;
+      stk.push(0u);
     }
     if (++steps > t.max_steps) {
       r.flags |= FLAG_ITERATION_GUARD;
       break;
     }
-    const uint32_t key = stk[head];
+    const uint32_t key = stk.s0;
     const NodeRec& n = (key < ntop ? top : t.nodes)[key];
     const uint64_t occ = n.occ;
     const uint32_t meta = n.meta;
@@ -305,7 +339,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     const V3f rel = (p - bounds.min) * 4.0f;
     V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
     bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
-    uint32_t flat = bitmap_index(bp, &r.flags);
+    uint32_t flat = bitmap_index_clamped(bp);
     const bool pop = backtrack || target == OOB_OCTANT || occ == 0ull ||
                      (occ & lut[flat * 8u + dir_idx]) == 0ull;
     Cube tb = child_bounds_for(bounds, target);
@@ -317,9 +351,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       // PUSH (:484-492)
       bounds = tb;
       target = hash_region(p - tb.min, tb.size / 2.0f);
-      head = (head + 1u) & 3u;
-      count = count < 4u ? count + 1u : 4u;
-      stk[head] = tck;
+      stk.push(tck);
       if (LOD) mip_level -= 1.0f;
       continue;
     }
@@ -333,12 +365,11 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       break;
     }
     V3f st{0.0f, 0.0f, 0.0f};
-    if (!pop || count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
+    if (!pop || stk.count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
     if (pop) {
-      head = (head + 3u) & 3u;
-      count -= 1u;
+      stk.pop();
       if (LOD) mip_level += 1.0f;
-      if (count != 0u) {
+      if (stk.count != 0u) {
         const V3f center = bounds.min + v3u(bounds.size / 2.0f);
         const float s2 = bounds.size * 2.0f;
         const float rcp_s2 = size_rcp<P2>(s2);
@@ -380,9 +411,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
   }
   return miss_result(r.flags);
 hit_found:
-  // the normal is derived once, after the loop, from the hit cell and point
   r.hit = 1;
-  r.normal = cube_impact_normal(r.cell, r.impact);
   return r;
 }
 
@@ -412,8 +441,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
   const float lo = 0.00001f;
   const float hi = 4.0f - 0.00001f;
 
-  uint32_t stk[4] = {0u, 0u, 0u, 0u};
-  uint32_t head = 1u, count = 1u;
+  RingStack stk{0u, 0u, 0u, 0u, 1u};  // the root pushed
   Cube bounds = root;
   V3f p;
   uint32_t target;
@@ -435,11 +463,11 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     // phase 1: stack-loop iterations up to the next due probe
     bool due = false;
     while (!finished) {
-      if (count == 0u) {
+      if (stk.count == 0u) {
         // restart from the root (:551-562)
         p = p + d * 0.1f;
-        target = (p.x < t.size && p.y < t.size && p.z < t.size && p.x > 0.0f && p.y > 0.0f &&
-                  p.z > 0.0f)
+        target = ((p.x < t.size) & (p.y < t.size) & (p.z < t.size) & (p.x > 0.0f) & (p.y > 0.0f) &
+                   (p.z > 0.0f))
                      ? hash_region(p, t.size / 2.0f)
                      : OOB_OCTANT;
         if (target == OOB_OCTANT) {
@@ -447,16 +475,14 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
           break;
         }
         bounds = root;
-        head = (head + 1u) & 3u;
-        count = 1u;
-        stk[head] = 0u;
+        stk.push(0u);
       }
       if (!probed && ++steps > t.max_steps) {
         r.flags |= FLAG_ITERATION_GUARD;
         finished = true;
         break;
       }
-      const uint32_t key = stk[head];
+      const uint32_t key = stk.s0;
       const NodeRec& n = (key < ntop ? top : t.nodes)[key];
       const uint64_t occ = n.occ;
       const uint32_t meta = n.meta;
@@ -472,13 +498,12 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       const V3f rel = (p - bounds.min) * 4.0f;
       V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
       bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
-      uint32_t flat = bitmap_index(bp, &r.flags);
+      uint32_t flat = bitmap_index_clamped(bp);
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
           (occ & lut[flat * 8u + dir_idx]) == 0ull) {
         // POP (:445-475)
-        head = (head + 3u) & 3u;
-        count -= 1u;
-        if (count != 0u) {
+        stk.pop();
+        if (stk.count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);
           const float s2 = bounds.size * 2.0f;
           const float rcp_s2 = size_rcp<P2>(s2);
@@ -500,9 +525,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
         // PUSH (:484-492)
         bounds = tb;
         target = hash_region(p - tb.min, tb.size / 2.0f);
-        head = (head + 1u) & 3u;
-        count = count < 4u ? count + 1u : 4u;
-        stk[head] = tck;
+        stk.push(tck);
       } else {
         // ADVANCE (:497-544)
         for (;;) {
@@ -529,7 +552,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     if (!due) continue;
     // phase 2: the due probe of the iteration, lanes of the wave together
     {
-      const NodeRec& n = (stk[head] < ntop ? top : t.nodes)[stk[head]];
+      const NodeRec& n = (stk.s0 < ntop ? top : t.nodes)[stk.s0];
       const uint32_t meta = n.meta;
       const bool uniform = node_kind(meta) == KIND_UNIFORM_LEAF;
       const uint32_t slot = uniform ? 0u : target;
@@ -537,7 +560,6 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
                           uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
                           &p, &r)) {
         r.hit = 1;
-        r.normal = cube_impact_normal(r.cell, r.impact);
         return r;
       }
       probed = true;
@@ -643,7 +665,6 @@ SVO_HD __attribute__((always_inline)) inline bool ray_step(const TreeView& t,
                            s.scale, &s.p, &hr)) {
       r = hr;
       r.hit = 1;
-      r.normal = cube_impact_normal(hr.cell, hr.impact);
       r.flags = s.flags | hr.flags;
       return true;
     }

@@ -80,9 +80,10 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int m
     h.impact[0] = c.impact.x;
     h.impact[1] = c.impact.y;
     h.impact[2] = c.impact.z;
-    h.normal[0] = c.normal.x;
-    h.normal[1] = c.normal.y;
-    h.normal[2] = c.normal.z;
+    const svo::V3f nrm = svo::hit_normal(c);
+    h.normal[0] = nrm.x;
+    h.normal[1] = nrm.y;
+    h.normal[2] = nrm.z;
   }
 }
 
