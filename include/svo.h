@@ -361,8 +361,11 @@ typedef struct svo_job {
 } svo_job;
 
 /* A reusable batch context for up to max_jobs jobs; its device buffers and
- * graph live until svo_batch_destroy. Runs on one context are ordered by
- * the caller; separate contexts may run concurrently on different streams. */
+ * graph live until svo_batch_destroy. Runs on one context execute in call
+ * order: a run waits, on its stream, for the context's previous run, and the
+ * host blocks only when it would refill the job table of the run two calls
+ * back before that run's copy has finished. Separate contexts may run
+ * concurrently on different streams. */
 int svo_batch_create(uint32_t max_jobs, svo_batch** out);
 void svo_batch_destroy(svo_batch* batch);
 /* Launches all jobs as one persistent kernel on `stream` (asynchronous). The
