@@ -147,11 +147,15 @@ SVO_HD inline CastResult miss_result(uint32_t flags) {
   return r;
 }
 
-// bitmap_index of a position clamped to [1e-5, 4 - 1e-5] (or NaN, which the
-// clamp passes through and the conversion maps to 0): every axis is already
-// in 0..3, so the range test is left out.
-SVO_HD inline uint32_t bitmap_index_clamped(V3f bp) {
-  return f2u32(floorf(bp.x)) + 4u * f2u32(floorf(bp.y)) + 16u * f2u32(floorf(bp.z));
+// bitmap_index(rclamp(bp, 1e-5, 4 - 1e-5)) without forming the clamped
+// position: per axis, floor then a saturating conversion (NaN -> 0) and a
+// clamp of the integer to 0..3 give the same cell as clamping first (below
+// 1e-5 and above 4 - 1e-5 the floor is <= 0 and >= 3; NaN passes the float
+// clamp and converts to 0 either way), and no range test is needed.
+SVO_HD inline uint32_t clamp03(int32_t v) { return static_cast<uint32_t>(v < 0 ? 0 : (v > 3 ? 3 : v)); }
+SVO_HD inline uint32_t bitmap_cell(V3f bp) {
+  return clamp03(f2i32(floorf(bp.x))) + 4u * clamp03(f2i32(floorf(bp.y))) +
+         16u * clamp03(f2i32(floorf(bp.z)));
 }
 
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
@@ -337,9 +341,11 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     }
     const float rcp_size = size_rcp<P2>(bounds.size);
     const V3f rel = (p - bounds.min) * 4.0f;
+    // the position in the node's 4x4x4 bitmap; the clamp to [lo, hi]
+    // (:425-436) is applied to the float position only where ADVANCE
+    // continues from it, the cell index comes from bitmap_cell
     V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
-    bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
-    uint32_t flat = bitmap_index_clamped(bp);
+    uint32_t flat = bitmap_cell(bp);
     const bool pop = backtrack || target == OOB_OCTANT || occ == 0ull ||
                      (occ & lut[flat * 8u + dir_idx]) == 0ull;
     Cube tb = child_bounds_for(bounds, target);
@@ -386,6 +392,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     // the rest of ADVANCE
     bool guard = false;
     target = step_octant(target, st);
+    bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
     while (target != OOB_OCTANT) {
       tb = child_bounds_for(bounds, target);
       const V3f sd = st * 4.0f;
@@ -497,8 +504,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       const float rcp_size = size_rcp<P2>(bounds.size);
       const V3f rel = (p - bounds.min) * 4.0f;
       V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
-      bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
-      uint32_t flat = bitmap_index_clamped(bp);
+      uint32_t flat = bitmap_cell(bp);  // cell of the clamped position
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
           (occ & lut[flat * 8u + dir_idx]) == 0ull) {
         // POP (:445-475)
@@ -527,7 +533,8 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
         target = hash_region(p - tb.min, tb.size / 2.0f);
         stk.push(tck);
       } else {
-        // ADVANCE (:497-544)
+        // ADVANCE (:497-544), from the clamped position
+        bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
         for (;;) {
           if (++steps > t.max_steps) {
             r.flags |= FLAG_ITERATION_GUARD;
