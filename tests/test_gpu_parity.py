@@ -161,6 +161,26 @@ def test_axis_aligned_and_degenerate_rays(gpu_device):
     assert_parity(exp, got, "axis aligned")
 
 
+def test_extreme_rays(gpu_device):
+    """NaN, infinite, huge, subnormal and zero ray components: the kernels'
+    hardware float-to-int conversions (saturating, NaN -> 0) and the power-of-
+    two arithmetic give the oracle's results, flags included, on every ray;
+    also towards a tree of brick dimension 1 and 32."""
+    nan, inf = float("nan"), float("inf")
+    origins = [(64.5, 200.0, 64.5), (30.0, 40.0, 50.0), (-1e30, 64.0, 64.0), (1e30, 1e30, 1e30),
+               (inf, 10.0, 10.0), (nan, 10.0, 10.0), (64.0, 64.0, 64.0), (-5.0, -5.0, -5.0),
+               (127.99999, 0.0, 64.0), (3e9, 5.0, 5.0)]
+    dirs = [(0, 0, 0), (nan, 0, -1), (0, nan, 0), (nan, nan, nan), (1e20, -1e20, 1e20),
+            (1e-30, -1.0, 1e-30), (1e-40, 1e-40, -1.0), (inf, -1, 0), (-inf, -inf, inf),
+            (0.6, -0.8, 0.0), (1, 1, 1), (-0.0, -0.0, 1.0), (-1.0, 0.0, 0.0)]
+    rays = np.array([[*o, *d] for o in origins for d in dirs], np.float32)
+    trees = [W.build_scene(W.PLAIN, seed=1), Octree.scene(1, 64, 1, 3), Octree.scene(2, 128, 32, 5)]
+    for i, tree in enumerate(trees):
+        _say(f"extreme rays, tree {i}")
+        got = _cast(tree, rays, gpu_device)
+        assert_parity(_oracle(tree).cast(rays), got, f"extreme rays, tree {i}")
+
+
 def test_empty_inputs(gpu_device):
     """Zero rays and an empty tree are valid inputs."""
     import torch

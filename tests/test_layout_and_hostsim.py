@@ -307,6 +307,28 @@ def test_p2_and_general_forms_agree():
             assert all(v == 0 for k, v in r.items() if k != "rays"), (size, dim, vd, r)
 
 
+def test_extreme_rays_on_the_host_build():
+    """NaN, infinite, huge, subnormal and zero ray components: the host build
+    of the kernels' traversal (both arrangements) equals the oracle on every
+    ray, towards trees of brick dimension 8, 1 and 32."""
+    nan, inf = float("nan"), float("inf")
+    origins = [(64.5, 200.0, 64.5), (30.0, 40.0, 50.0), (-1e30, 64.0, 64.0), (1e30, 1e30, 1e30),
+               (inf, 10.0, 10.0), (nan, 10.0, 10.0), (64.0, 64.0, 64.0), (-5.0, -5.0, -5.0),
+               (127.99999, 0.0, 64.0), (3e9, 5.0, 5.0)]
+    dirs = [(0, 0, 0), (nan, 0, -1), (0, nan, 0), (nan, nan, nan), (1e20, -1e20, 1e20),
+            (1e-30, -1.0, 1e-30), (1e-40, 1e-40, -1.0), (inf, -1, 0), (-inf, -inf, inf),
+            (0.6, -0.8, 0.0), (1, 1, 1), (-0.0, -0.0, 1.0), (-1.0, 0.0, 0.0)]
+    rays = np.array([[*o, *d] for o in origins for d in dirs], np.float32)
+    for tree in (W.build_scene(W.PLAIN, seed=1), Octree.scene(1, 64, 1, 3),
+                 Octree.scene(2, 128, 32, 5)):
+        a = tree.flatten().arrays()
+        exp = oracle.OracleTree(tree.export_model()).cast(rays)
+        for regrouped in (False, True):
+            got = hostsim.cast(a, rays, regrouped=regrouped)
+            res = compare(exp, {k: got[k] for k in got.dtype.names})
+            assert all(v == 0 for k, v in res.items() if k != "rays"), (regrouped, res)
+
+
 def test_regrouped_traversal_under_forced_guard():
     """cast_ray_ww counts iterations exactly as cast_ray (an iteration
     resumed after its probe is not counted twice): with lowered bounds both
@@ -343,3 +365,17 @@ def test_forced_iteration_guard_reports_and_ends_rays():
         assert all(v == 0 for k, v in res.items() if k != "rays"), (limit, res)
     # the default bound is never reached on this workload
     assert (hostsim.cast(a, rays)["flags"] == 0).all()
+
+
+def test_parity_comparison_itself():
+    """The comparison used by every parity test: a NaN equals any NaN (sign
+    and payload of generated NaNs are platform-defined), while a NaN against a
+    number and a one-ulp difference both count as mismatches."""
+    dt = np.dtype([("hit", "<u4"), ("impact", "<f4", (3,))])
+    e = np.zeros(3, dt)
+    e["hit"] = 1
+    e["impact"] = [[np.nan, 1, 2], [1, 2, 3], [np.nan, 0, 0]]
+    g = {"hit": e["hit"].copy(),
+         "impact": np.array([[-np.nan, 1, 2], [1, 2, np.nextafter(np.float32(3), np.float32(4))],
+                             [1.0, 0, 0]], np.float32)}
+    assert compare(e, g) == {"rays": 3, "hit": 0, "impact": 2}
