@@ -217,6 +217,24 @@ SVO_HD inline V3f dda_step(V3f dir, V3f sig, V3f* p, Cube b, V3f scale) {
   return V3f{m == dx ? sig.x : 0.0f, m == dy ? sig.y : 0.0f, m == dz ? sig.z : 0.0f};
 }
 
+// dda_step reporting which axes stepped (m == d{x,y,z}, the axes whose step
+// component is signum(direction)); the others' components are +0.0.
+SVO_HD inline void dda_step_axes(V3f dir, V3f sig, V3f* p, Cube b, V3f scale, bool* ax, bool* ay,
+                                 bool* az) {
+  const V3f diff = *p - b.min;
+  const float snx = b.size * rmax(sig.x, 0.0f) - sig.x * diff.x;
+  const float sny = b.size * rmax(sig.y, 0.0f) - sig.y * diff.y;
+  const float snz = b.size * rmax(sig.z, 0.0f) - sig.z * diff.z;
+  const float dx = fabsf(snx * scale.x);
+  const float dy = fabsf(sny * scale.y);
+  const float dz = fabsf(snz * scale.z);
+  const float m = fminf(fminf(dx, dy), dz);
+  *p = *p + dir * m;
+  *ax = m == dx;
+  *ay = m == dy;
+  *az = m == dz;
+}
+
 // Exact reciprocal of a power of two, 0 for any other value. For s = 2^k the
 // products x * 2^-k and the quotient x / 2^k are the same real number, and
 // both operations round it once to nearest-even, so `x * pow2_rcp(s)` is

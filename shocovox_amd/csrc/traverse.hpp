@@ -186,6 +186,11 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
                        unit,
           unit};
   const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
+  // A step moves the cell by signum(direction) on the axes that stepped and
+  // leaves the others (their step component is +0.0, and cell corners are
+  // never -0.0): the per-axis moves are fixed for the whole probe.
+  const V3f umove{sig.x * unit, sig.y * unit, sig.z * unit};
+  const int mx = step_int(sig.x), my = step_int(sig.y), mz = step_int(sig.z);
   for (int guard = 0; guard < 4 * dim + 16; ++guard) {
     if constexpr (P2) {
       // with dim a power of two, all three are in [0, dim) exactly when
@@ -208,11 +213,13 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
       r->cell = hb;
       return true;
     }
-    const V3f step = dda_step(d, sig, p, cb, scale);
-    cb.min = cb.min + step * unit;
-    ix += step_int(step.x);
-    iy += step_int(step.y);
-    iz += step_int(step.z);
+    bool sx, sy, sz;
+    dda_step_axes(d, sig, p, cb, scale, &sx, &sy, &sz);
+    cb.min = V3f{sx ? cb.min.x + umove.x : cb.min.x, sy ? cb.min.y + umove.y : cb.min.y,
+                 sz ? cb.min.z + umove.z : cb.min.z};
+    ix += sx ? mx : 0;
+    iy += sy ? my : 0;
+    iz += sz ? mz : 0;
   }
   r->flags |= FLAG_ITERATION_GUARD;
   return false;
