@@ -349,7 +349,10 @@ typedef struct svo_batch svo_batch;
 
 /* One job: kind 0 renders `viewport` at width x height (rays generated on
  * the device, as svo_render); kind 1 casts `width` rays from `rays` (as
- * svo_cast). Results go to `out` exactly as the single-job calls write them. */
+ * svo_cast). With viewing_distance > 0 the job is get_by_ray_at_lod (as
+ * svo_render_at_lod / svo_cast_at_lod); 0 is get_by_ray; negative or NaN is
+ * SVO_INVALID_ARGUMENT. Results go to `out` exactly as the single-job calls
+ * write them. */
 typedef struct svo_job {
   const svo_tree* tree;
   uint32_t kind;
@@ -358,6 +361,7 @@ typedef struct svo_job {
   svo_viewport viewport;
   const float* rays;
   svo_hits out;
+  float viewing_distance;
 } svo_job;
 
 /* A reusable batch context for up to max_jobs jobs; its device buffers and

@@ -67,7 +67,7 @@ class Viewport(C.Structure):
 class Job(C.Structure):
     _fields_ = [("tree", C.c_void_p), ("kind", C.c_uint32), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("viewport", Viewport), ("rays", C.c_void_p),
-                ("out", Hits)]
+                ("out", Hits), ("viewing_distance", C.c_float)]
 
 
 class ServeConfig(C.Structure):

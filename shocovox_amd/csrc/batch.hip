@@ -50,7 +50,8 @@ struct alignas(16) BatchJob {
   uint32_t tile_count;
   uint32_t first_tile;  // render: first image tile of the job
   uint32_t tile_major;  // 1: outputs stored tile after tile from out[0]
-  uint32_t pad[3];
+  float viewing_distance;  // > 0: get_by_ray_at_lod with this distance
+  uint32_t pad[2];
   Camera cam;
   const float* rays;
   HitPtrs out;
@@ -97,7 +98,10 @@ __device__ inline const TreeView* trees_of(const char* table) {
   return reinterpret_cast<const TreeView*>(table + header_of(table)->trees_offset);
 }
 
-template <int MIN_WAVES, bool WW = false>
+// LOD compiles in get_by_ray_at_lod for jobs with a viewing distance; a
+// context launches that form only for runs that hold such jobs, so the plain
+// form keeps its smaller register allocation.
+template <int MIN_WAVES, bool WW = false, bool LOD = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters) {
   const BatchHeader* hdr = header_of(table);
@@ -145,7 +149,12 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
       }
       if (valid) {
         CastResult r;
-        if constexpr (WW)
+        if constexpr (LOD) {
+          if (job.viewing_distance > 0.0f)
+            r = cast_ray_lod<true>(tv, tv.ray_lut, o, d, job.viewing_distance);
+          else
+            r = cast_ray<true>(tv, o, d);
+        } else if constexpr (WW)
           r = cast_ray_ww<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(nullptr), 0u);
         else
           r = cast_ray<true>(tv, o, d);
@@ -218,6 +227,8 @@ const BatchVariant kBatchVariants[] = {
     {"ww-w5-g5", batch_kernel<5, true>, 5},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+// runs that hold get_by_ray_at_lod jobs use this form in every variant
+constexpr BatchFn kBatchLodKernel = batch_kernel<1, false, true>;
 int g_batch_variant = 0;
 
 int hip_fail(hipError_t e, const char* what, std::string* err) {
@@ -246,6 +257,11 @@ struct BatchContext {
   hipGraph_t graph = nullptr;
   hipGraphNode_t copy_node = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the same sequence with the LOD form of the kernel, built on the first
+  // run that holds a job with a viewing distance
+  hipGraph_t graph_lod = nullptr;
+  hipGraphNode_t copy_node_lod = nullptr;
+  hipGraphExec_t exec_lod = nullptr;
   hipEvent_t done[2] = {nullptr, nullptr};  // end of the last run that read host[i]
   bool pending[2] = {false, false};
   uint64_t last_tiles = 0;
@@ -256,10 +272,11 @@ struct BatchContext {
 
 namespace {
 
-int build_graph(BatchContext* b, std::string* err) {
-  hipError_t e = hipGraphCreate(&b->graph, 0);
+int build_graph(BatchContext* b, BatchFn fn, hipGraph_t* graph, hipGraphNode_t* copy_node,
+                hipGraphExec_t* exec, std::string* err) {
+  hipError_t e = hipGraphCreate(graph, 0);
   if (e != hipSuccess) return hip_fail(e, "hipGraphCreate", err);
-  e = hipGraphAddMemcpyNode1D(&b->copy_node, b->graph, nullptr, 0, b->dev, b->host[0],
+  e = hipGraphAddMemcpyNode1D(copy_node, *graph, nullptr, 0, b->dev, b->host[0],
                               b->table_bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipGraphAddMemcpyNode1D", err);
   b->arg_table = static_cast<const char*>(b->dev);
@@ -273,19 +290,19 @@ int build_graph(BatchContext* b, std::string* err) {
   rp.sharedMemBytes = 0;
   rp.kernelParams = reset_args;
   rp.extra = nullptr;
-  e = hipGraphAddKernelNode(&reset_node, b->graph, &b->copy_node, 1, &rp);
+  e = hipGraphAddKernelNode(&reset_node, *graph, copy_node, 1, &rp);
   if (e != hipSuccess) return hip_fail(e, "hipGraphAddKernelNode(reset)", err);
   void* args[] = {&b->arg_table, &b->arg_counters};
   hipKernelNodeParams kp{};
-  kp.func = reinterpret_cast<void*>(kBatchVariants[b->variant].fn);
+  kp.func = reinterpret_cast<void*>(fn);
   kp.gridDim = dim3(b->grid);
   kp.blockDim = dim3(256);
   kp.sharedMemBytes = 0;
   kp.kernelParams = args;
   kp.extra = nullptr;
-  e = hipGraphAddKernelNode(&kernel_node, b->graph, &reset_node, 1, &kp);
+  e = hipGraphAddKernelNode(&kernel_node, *graph, &reset_node, 1, &kp);
   if (e != hipSuccess) return hip_fail(e, "hipGraphAddKernelNode(batch)", err);
-  e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
   if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate", err);
   return SVO_OK;
 }
@@ -333,7 +350,8 @@ int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
   std::memset(b->host[1], 0, b->table_bytes);
   // groups_per_cu 256-lane workgroups per CU: that many waves on every SIMD
   b->grid = static_cast<uint32_t>(std::max(cus, 1)) * kBatchVariants[b->variant].groups_per_cu;
-  const int st = build_graph(b, err);
+  const int st = build_graph(b, kBatchVariants[b->variant].fn, &b->graph, &b->copy_node,
+                             &b->exec, err);
   if (st != SVO_OK) {
     batch_destroy(b);
     return st;
@@ -349,6 +367,8 @@ void batch_destroy(BatchContext* b) {
     if (b->pending[i]) (void)hipEventSynchronize(b->done[i]);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->exec_lod) (void)hipGraphExecDestroy(b->exec_lod);
+  if (b->graph_lod) (void)hipGraphDestroy(b->graph_lod);
   for (int i = 0; i < 2; ++i)
     if (b->done[i]) (void)hipEventDestroy(b->done[i]);
   if (b->counters) (void)hipFree(b->counters);
@@ -381,6 +401,7 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
   TreeView* trees = reinterpret_cast<TreeView*>(base + trees_offset);
   std::unordered_map<const void*, uint32_t> tree_index;
   uint64_t tiles = 0;
+  bool any_lod = false;
   for (uint32_t i = 0; i < n; ++i) {
     const BatchJobIn& in = jobs[i];
     if (!in.tree || in.tree->device != b->device) {
@@ -426,6 +447,12 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     }
     j.tile_begin = static_cast<uint32_t>(tiles);
     j.out = to_ptrs(in.out);
+    if (!(in.viewing_distance >= 0.0f)) {
+      *err = "viewing distance must be >= 0 (0: get_by_ray)";
+      return SVO_INVALID_ARGUMENT;
+    }
+    j.viewing_distance = in.viewing_distance;
+    any_lod = any_lod || in.viewing_distance > 0.0f;
     tiles += j.tile_count;
     if (tiles >= (1ull << 26)) {
       *err = "too many tiles in one batch (at most 2^26 tiles of 64 rays)";
@@ -438,15 +465,21 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
   hdr->trees_offset = static_cast<uint32_t>(trees_offset);
   b->last_tiles = tiles;
   const size_t used = trees_offset + tree_index.size() * sizeof(TreeView);
-  hipError_t e = hipGraphExecMemcpyNodeSetParams1D(b->exec, b->copy_node, b->dev, base, used,
-                                                   hipMemcpyHostToDevice);
+  if (any_lod && !b->exec_lod) {
+    const int st = build_graph(b, kBatchLodKernel, &b->graph_lod, &b->copy_node_lod,
+                               &b->exec_lod, err);
+    if (st != SVO_OK) return st;
+  }
+  hipGraphExec_t exec = any_lod ? b->exec_lod : b->exec;
+  hipError_t e = hipGraphExecMemcpyNodeSetParams1D(
+      exec, any_lod ? b->copy_node_lod : b->copy_node, b->dev, base, used, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipGraphExecMemcpyNodeSetParams1D", err);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (b->pending[slot ^ 1u]) {
     e = hipStreamWaitEvent(s, b->done[slot ^ 1u], 0);
     if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent", err);
   }
-  e = hipGraphLaunch(b->exec, s);
+  e = hipGraphLaunch(exec, s);
   if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch", err);
   e = hipEventRecord(b->done[slot], s);
   if (e != hipSuccess) return hip_fail(e, "hipEventRecord", err);

@@ -26,6 +26,8 @@ struct BatchJobIn {
   uint32_t first_tile = 0;
   uint32_t tile_limit = 0;
   uint32_t tile_major = 0;
+  // > 0: get_by_ray_at_lod with this viewing distance; 0: get_by_ray
+  float viewing_distance = 0.0f;
 };
 
 int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err);

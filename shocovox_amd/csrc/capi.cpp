@@ -653,6 +653,7 @@ int svo_batch_run(svo_batch* batch, const svo_job* jobs, uint32_t n, void* strea
     b.height = j.height;
     b.rays = j.rays;
     b.out = j.out;
+    b.viewing_distance = j.viewing_distance;
     if (j.kind == 0)
       b.cam = svo::make_camera(
           svo::V3f{j.viewport.origin[0], j.viewport.origin[1], j.viewport.origin[2]},

@@ -597,13 +597,17 @@ class Batch:
             _release("svo_batch_destroy", h)
 
     @staticmethod
-    def render_job(tree: "DeviceTree", vp: "Viewport", w: int, h: int, bufs: dict) -> N.Job:
-        return N.Job(tree._h, 0, w, h, vp.to_c(), None, _hits_struct(bufs))
+    def render_job(tree: "DeviceTree", vp: "Viewport", w: int, h: int, bufs: dict,
+                   viewing_distance: float = 0.0) -> N.Job:
+        """A viewport render; with viewing_distance > 0, get_by_ray_at_lod."""
+        return N.Job(tree._h, 0, w, h, vp.to_c(), None, _hits_struct(bufs),
+                     float(viewing_distance))
 
     @staticmethod
-    def cast_job(tree: "DeviceTree", rays, bufs: dict) -> N.Job:
+    def cast_job(tree: "DeviceTree", rays, bufs: dict, viewing_distance: float = 0.0) -> N.Job:
+        """A ray-array cast; with viewing_distance > 0, get_by_ray_at_lod."""
         return N.Job(tree._h, 1, rays.shape[0], 1, N.Viewport(), rays.data_ptr(),
-                     _hits_struct(bufs))
+                     _hits_struct(bufs), float(viewing_distance))
 
     def run(self, jobs: list, stream=None) -> None:
         arr = (N.Job * max(1, len(jobs)))(*jobs)

@@ -512,3 +512,50 @@ def test_lod_traversal_on_device(gpu_device, vd):
         ents[:, 1] = 0x8844CCFF
         tree.insert_many(xyz, ents)
         d.update(tree)
+
+
+def test_batched_lod_jobs(gpu_device):
+    """get_by_ray_at_lod in the batched mode: render and cast jobs with
+    viewing distances (and plain jobs beside them, in one run) on trees with
+    MIPs enabled equal the oracle; a plain run on the same context afterwards
+    still does; a negative viewing distance is refused."""
+    import torch
+    from shocovox_amd import Batch, alloc_hits
+    from shocovox_amd._native import SvoError
+    trees = [Octree.scene(2, 256, 8, 12), Octree.scene(1, 64, 4, 3)]
+    for t in trees:
+        t.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    dtrees = [DeviceTree(t) for t in trees]
+    oracles = [_oracle(t) for t in trees]
+    fields = ("hit", "pix", "albedo", "voxel", "distance", "impact", "normal", "flags")
+    vp = W.look_at_viewport((330.0, 240.0, 300.0), (128.0, 128.0, 128.0), 200, 136)
+    prim = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 200, 136)
+    inc = W.incoherent_rays(64, 30_000, seed=5)
+    inc_dev = torch.from_numpy(inc).to(gpu_device)
+    batch = Batch(8)
+    cases = [("render", 0, 2.0), ("render", 0, 0.0), ("cast", 1, 0.5), ("cast", 1, 0.0),
+             ("render", 0, 40.0)]
+    bufs, jobs = [], []
+    for kind, ti, vd in cases:
+        if kind == "render":
+            b = alloc_hits(200 * 136, gpu_device, fields)
+            jobs.append(Batch.render_job(dtrees[ti], vp, 200, 136, b, viewing_distance=vd))
+        else:
+            b = alloc_hits(len(inc), gpu_device, fields)
+            jobs.append(Batch.cast_job(dtrees[ti], inc_dev, b, viewing_distance=vd))
+        bufs.append(b)
+    for rnd, sel in enumerate((list(range(len(cases))), [1, 3])):
+        _say(f"batched lod run {rnd}")
+        for b in bufs:
+            for k in fields:
+                b[k].zero_()
+        batch.run([jobs[i] for i in sel])
+        torch.cuda.synchronize()
+        for i in sel:
+            kind, ti, vd = cases[i]
+            rays = prim if kind == "render" else inc
+            exp = oracles[ti].cast(rays, viewing_distance=vd) if vd > 0 else oracles[ti].cast(rays)
+            assert_parity(exp, gpu_to_numpy(bufs[i]), f"batched {kind} vd={vd} run {rnd}")
+    bad = Batch.cast_job(dtrees[1], inc_dev, bufs[3], viewing_distance=-1.0)
+    with pytest.raises(SvoError):
+        batch.run([bad])
