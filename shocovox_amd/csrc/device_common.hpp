@@ -28,8 +28,12 @@ __device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint32_
   if (o.hit) o.hit[i] = r.hit;
   if (o.pix) o.pix[i] = r.hit ? r.pix : EMPTY_MARKER;
   if (o.albedo) {
-    const uint32_t c = pix_color_index(r.pix);
-    o.albedo[i] = (r.hit && c < NONE16 && c < t.color_count) ? t.colors[c] : 0u;
+    uint32_t a = 0u;
+    if (r.hit) {
+      const uint32_t c = pix_color_index(r.pix);
+      if (c < NONE16 && c < t.color_count) a = t.colors[c];
+    }
+    o.albedo[i] = a;
   }
   if (o.voxel) {
     if (r.hit) {
@@ -44,9 +48,9 @@ __device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint32_
   }
   if (o.distance) o.distance[i] = r.hit ? length(r.impact - org) : __builtin_inff();
   if (o.impact) {
-    o.impact[3 * i + 0] = r.impact.x;
-    o.impact[3 * i + 1] = r.impact.y;
-    o.impact[3 * i + 2] = r.impact.z;
+    o.impact[3 * i + 0] = r.hit ? r.impact.x : 0.0f;
+    o.impact[3 * i + 1] = r.hit ? r.impact.y : 0.0f;
+    o.impact[3 * i + 2] = r.hit ? r.impact.z : 0.0f;
   }
   if (o.normal) {
     const V3f nrm = hit_normal(r);

@@ -98,6 +98,8 @@ SVO_HD inline float sub_fmod_size(float a, float b, float rcp_b) {
 
 // A ray's result. The normal of a hit is a function of the hit cell and the
 // impact point (hit_normal); it is derived only where a caller asks for it.
+// For a miss only `hit` (0) and `flags` are defined; consumers write the
+// miss values themselves (see miss_result for what a miss reports).
 struct CastResult {
   uint32_t hit;
   uint32_t pix;
@@ -136,7 +138,18 @@ SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
   return x + 4u * y + 16u * z;
 }
 
-// A finished ray that hit nothing, as cast_ray returns it.
+// A ray that hit nothing: the only defined fields are hit and flags.
+// (Materialising the miss values here would make the compiler set them in
+// every loop iteration, ahead of the loop's exits.)
+SVO_HD inline CastResult miss_flags(uint32_t flags) {
+  CastResult r;
+  r.hit = 0;
+  r.flags = flags;
+  return r;
+}
+
+// A miss with the values get_by_ray's callers see: no palette value, impact
+// and cell zero.
 SVO_HD inline CastResult miss_result(uint32_t flags) {
   CastResult r;
   r.hit = 0;
@@ -268,7 +281,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop, float viewing_distance) {
   // Only the flags live across the loops; the probe fills pix, impact and
-  // cell on the hit path, and a miss returns miss_result(flags). (Zeroing
+  // cell on the hit path, and a miss returns miss_flags(flags). (Zeroing
   // the hit fields up front would make the compiler carry them around every
   // loop.)
   CastResult r;
@@ -296,7 +309,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       target = OOB_OCTANT;
     }
   }
-  if (target == OOB_OCTANT) return miss_result(0u);
+  if (target == OOB_OCTANT) return miss_flags(0u);
   uint32_t steps = 0;
   // log2(size / dim) of two powers of two, exactly
   float mip_level =
@@ -358,13 +371,13 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     Cube tb = child_bounds_for(bounds, target);
     // the child word is read only when the occupancy test passes (pure
     // reads: the same decision as testing the child first)
-    uint32_t tck = INVALID_INDEX;
-    if (!pop && kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull) tck = tword;
-    if (tck != INVALID_INDEX) {
+    const bool push = !pop && kind == KIND_INTERNAL &&
+                      (occ & octant_bitmap_mask(target)) != 0ull && tword != INVALID_INDEX;
+    if (push) {
       // PUSH (:484-492)
       bounds = tb;
       target = hash_region(p - tb.min, tb.size / 2.0f);
-      stk.push(tck);
+      stk.push(tword);
       if (LOD) mip_level -= 1.0f;
       continue;
     }
@@ -377,7 +390,8 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       r.flags |= FLAG_ITERATION_GUARD;
       break;
     }
-    V3f st{0.0f, 0.0f, 0.0f};
+    // (left unset when the POP empties the stack: then it is not read)
+    V3f st;
     if (!pop || stk.count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
     if (pop) {
       stk.pop();
@@ -423,7 +437,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       break;
     }
   }
-  return miss_result(r.flags);
+  return miss_flags(r.flags);
 hit_found:
   r.hit = 1;
   return r;

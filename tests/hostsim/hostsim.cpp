@@ -66,7 +66,7 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int m
     h.hit = c.hit;
     h.flags = c.flags;
     h.pix = c.hit ? c.pix : 0xFFFFFFFFu;
-    const uint32_t ci = c.pix & 0xFFFFu;
+    const uint32_t ci = c.hit ? (c.pix & 0xFFFFu) : 0xFFFFu;
     h.albedo = (c.hit && ci < 0xFFFFu && ci < t->color_count) ? t->colors[ci] : 0u;
     if (c.hit) {
       h.voxel[0] = svo::hit_voxel_axis(c.impact.x, c.cell.min.x, c.cell.size);
@@ -77,9 +77,9 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int m
       h.voxel[0] = h.voxel[1] = h.voxel[2] = 0xFFFFFFFFu;
       h.distance = __builtin_inff();
     }
-    h.impact[0] = c.impact.x;
-    h.impact[1] = c.impact.y;
-    h.impact[2] = c.impact.z;
+    h.impact[0] = c.hit ? c.impact.x : 0.0f;
+    h.impact[1] = c.hit ? c.impact.y : 0.0f;
+    h.impact[2] = c.hit ? c.impact.z : 0.0f;
     const svo::V3f nrm = svo::hit_normal(c);
     h.normal[0] = nrm.x;
     h.normal[1] = nrm.y;
