@@ -16,6 +16,13 @@
 // reference loop has no termination proof) and a bitmap-index range check
 // (the reference would index its LUT out of bounds and panic). Either sets a
 // flag on the ray; on well-formed inputs both stay clear.
+//
+// What differs from the reference is only how the same operations are laid
+// out for 64-lane waves: the restart loop and the stack loop are one loop;
+// POP and ADVANCE lanes share their first DDA step; the ring stack is a
+// shift register; a miss carries only its flags; cast_ray_ww regroups a
+// wave's brick probes; and trees whose sizes are powers of two (all device
+// trees) use the exact power-of-two forms of the size arithmetic (P2).
 #pragma once
 
 #include "layout.hpp"
