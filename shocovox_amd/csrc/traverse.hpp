@@ -466,7 +466,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop) {
   // (here, unlike in cast_ray_impl, zeroing the hit fields up front gives
-  // the smaller register allocation)
+  // the smaller register allocation, and measured 4% faster)
   CastResult r = miss_result(0u);
 
   const V3f scale = dda_scale_factors(d);
