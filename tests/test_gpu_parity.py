@@ -351,6 +351,39 @@ def test_batched_mixed_jobs_and_replays(gpu_device, variant):
     torch.cuda.synchronize()
 
 
+def test_batched_runs_back_to_back_on_two_streams(gpu_device):
+    """Runs issued without any host wait in between, alternating between two
+    streams, on one context: each run's table comes from the other pinned
+    staging, and a run waits on its stream for the previous one, so every
+    run's outputs (separate buffers per run) equal the oracle."""
+    import torch
+    from shocovox_amd import Batch, alloc_hits
+    tree = W.build_scene(W.PLAIN, seed=3)
+    d = DeviceTree(tree)
+    o = _oracle(tree)
+    streams = [torch.cuda.Stream(device=gpu_device), torch.cuda.Stream(device=gpu_device)]
+    batch = Batch(4)
+    runs = []
+    for k in range(6):
+        w, h = 96 + 16 * k, 64 + 8 * k
+        vp = W.look_at_viewport((200.0 - 10 * k, 150.0, 180.0 + 5 * k), (64.0, 40.0, 64.0), w, h)
+        bufs = alloc_hits(w * h, gpu_device)
+        rays = W.incoherent_rays(128, 500 + 100 * k, seed=40 + k)
+        t = torch.from_numpy(rays).to(gpu_device)
+        cbufs = alloc_hits(len(rays), gpu_device)
+        runs.append((vp, w, h, bufs, rays, t, cbufs))
+    torch.cuda.synchronize()
+    _say("back-to-back runs on two streams")
+    for k, (vp, w, h, bufs, rays, t, cbufs) in enumerate(runs):
+        batch.run([Batch.render_job(d, vp, w, h, bufs), Batch.cast_job(d, t, cbufs)],
+                  streams[k % 2])
+    torch.cuda.synchronize()
+    for k, (vp, w, h, bufs, rays, t, cbufs) in enumerate(runs):
+        prim = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, w, h)
+        assert_parity(o.cast(prim), gpu_to_numpy(bufs), f"run {k} render")
+        assert_parity(o.cast(rays), gpu_to_numpy(cbufs), f"run {k} cast")
+
+
 def test_spheres_2048_full_size(gpu_device):
     """The 2048^3 / dim 32 spheres workload at full size: 3840x2160 primary
     rays and 8M incoherent rays on the GPU. Every ray: no guard flag, every
