@@ -245,6 +245,27 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   return false;
 }
 
+// The rest of a ray whose POP of the root empties the stack because the
+// root has no occupied bitmap cell ahead of the ray (occupancy & LUT == 0).
+// The reference then restarts from the root at p + 0.1 d, fails the same
+// test, and so on until p leaves the tree: the LUT box of a cell holds the
+// boxes of every cell further along the ray (each coordinate moves towards
+// the box's far corner or stays), so no later position passes the test and
+// the ray ends as a miss. Only the nudges and the iteration count remain,
+// replayed here exactly (the restart's interior test, then one counted
+// iteration per restart). Not for MIP-enabled LOD rays (their MIP test runs
+// on every iteration) nor for a leaf root (its brick probe does).
+SVO_HD inline CastResult crawl_out_of_root(const TreeView& t, V3f p, V3f d, uint32_t steps,
+                                           uint32_t flags) {
+  for (;;) {
+    p = p + d * 0.1f;
+    if (!((p.x < t.size) & (p.y < t.size) & (p.z < t.size) & (p.x > 0.0f) & (p.y > 0.0f) &
+          (p.z > 0.0f)))
+      return miss_flags(flags);
+    if (++steps > t.max_steps) return miss_flags(flags | FLAG_ITERATION_GUARD);
+  }
+}
+
 // The reference's NodeStack<u32, 4> (raytracing_on_cpu.rs:19-82): a ring of
 // four whose push overwrites the oldest entry once full and whose count
 // says how many entries are retained. Kept here as a shift register, s0 the
@@ -401,6 +422,9 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     V3f st;
     if (!pop || stk.count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
     if (pop) {
+      if (stk.count == 1u && key == 0u && !backtrack && target != OOB_OCTANT &&
+          kind != KIND_LEAF && !(LOD && t.mips_enabled))
+        return crawl_out_of_root(t, p, d, steps, r.flags);
       stk.pop();
       if (LOD) mip_level += 1.0f;
       if (stk.count != 0u) {
@@ -536,6 +560,9 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
           (occ & lut[flat * 8u + dir_idx]) == 0ull) {
         // POP (:445-475)
+        if (stk.count == 1u && key == 0u && !backtrack && target != OOB_OCTANT &&
+            kind != KIND_LEAF)
+          return crawl_out_of_root(t, p, d, steps, r.flags);
         stk.pop();
         if (stk.count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);

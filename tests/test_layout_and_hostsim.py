@@ -329,6 +329,33 @@ def test_extreme_rays_on_the_host_build():
             assert all(v == 0 for k, v in res.items() if k != "rays"), (regrouped, res)
 
 
+def test_rays_that_leave_the_content_behind():
+    """Rays with nothing ahead of them at the root level (an empty tree, rays
+    passing beside the content of a sparse scene) make the reference restart
+    from the root at p + 0.1 d until they leave the tree; the kernels replay
+    only those nudges and the iteration count (crawl_out_of_root). Both
+    arrangements equal the oracle, which runs the full restarts, and agree
+    with each other under lowered iteration bounds (where the replayed count
+    decides the guard flag)."""
+    for tree in (Octree.new(64, 4), Octree.new(512, 16), Octree.scene(1, 256, 16, 3)):
+        size = tree.get_size()
+        a = tree.flatten().arrays()
+        rays = W.incoherent_rays(size, 1500, seed=3)
+        vp = W.look_at_viewport((1.3 * size, 0.9 * size, 1.2 * size), (size / 2,) * 3, 48, 32)
+        prim = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 48, 32)
+        o = oracle.OracleTree(tree.export_model())
+        for r in (rays, prim):
+            exp = o.cast(r)
+            for regrouped in (False, True):
+                got = hostsim.cast(a, r, regrouped=regrouped)
+                res = compare(exp, {k: got[k] for k in got.dtype.names})
+                assert all(v == 0 for k, v in res.items() if k != "rays"), (size, regrouped, res)
+        for limit in (7, 300, 3000):
+            g1 = hostsim.cast(a, rays, max_steps=limit)
+            g2 = hostsim.cast(a, rays, max_steps=limit, regrouped=True)
+            assert g1.tobytes() == g2.tobytes(), (size, limit)
+
+
 def test_regrouped_traversal_under_forced_guard():
     """cast_ray_ww counts iterations exactly as cast_ray (an iteration
     resumed after its probe is not counted twice): with lowered bounds both
