@@ -560,9 +560,8 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
           (occ & lut[flat * 8u + dir_idx]) == 0ull) {
         // POP (:445-475)
-        if (stk.count == 1u && key == 0u && !backtrack && target != OOB_OCTANT &&
-            kind != KIND_LEAF)
-          return crawl_out_of_root(t, p, d, steps, r.flags);
+        // (no crawl_out_of_root shortcut in this form: it costs 12 VGPRs
+        // here, which spills the 6-waves-per-SIMD builds)
         stk.pop();
         if (stk.count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);
