@@ -592,3 +592,38 @@ def test_batched_lod_jobs(gpu_device):
     bad = Batch.cast_job(dtrees[1], inc_dev, bufs[3], viewing_distance=-1.0)
     with pytest.raises(SvoError):
         batch.run([bad])
+
+
+def test_rays_that_leave_the_content_behind(gpu_device):
+    """Rays with nothing ahead of them at the root (an empty tree, rays beside
+    the content of a sparse scene): the batch kernel replays only the
+    reference's restart nudges (crawl_out_of_root), the while-while form used
+    by svo_render and svo_cast runs the full restarts; both equal the oracle,
+    in one batched run and as plain launches."""
+    import torch
+    from shocovox_amd import Batch, alloc_hits
+    fields = ("hit", "pix", "albedo", "voxel", "distance", "impact", "normal", "flags")
+    w, h = 160, 96
+    batch = Batch(8)
+    trees = (Octree.new(512, 16), Octree.scene(1, 256, 16, 3), Octree.new(64, 4))
+    for i, tree in enumerate(trees):
+        size = tree.get_size()
+        vp = W.look_at_viewport((1.3 * size, 0.9 * size, 1.2 * size), (size / 2,) * 3, w, h)
+        prim = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, w, h)
+        inc = W.incoherent_rays(size, 20_000, seed=3 + i)
+        o = _oracle(tree)
+        exp_p, exp_i = o.cast(prim), o.cast(inc)
+        d = DeviceTree(tree)
+        _say(f"leave-behind tree {i} (size {size}): batched run")
+        bp = alloc_hits(w * h, gpu_device, fields)
+        bi = alloc_hits(len(inc), gpu_device, fields)
+        inc_dev = torch.from_numpy(inc).to(gpu_device)
+        batch.run([Batch.render_job(d, vp, w, h, bp), Batch.cast_job(d, inc_dev, bi)])
+        torch.cuda.synchronize()
+        assert_parity(exp_p, gpu_to_numpy(bp), f"batched render, tree {i}")
+        assert_parity(exp_i, gpu_to_numpy(bi), f"batched cast, tree {i}")
+        _say(f"leave-behind tree {i}: plain launches")
+        got = d.render(vp, w, h)
+        torch.cuda.synchronize()
+        assert_parity(exp_p, gpu_to_numpy(got), f"plain render, tree {i}")
+        assert_parity(exp_i, _cast(tree, inc, gpu_device), f"plain cast, tree {i}")
