@@ -92,28 +92,33 @@ class Reader {
     t->assign(s, l);
     return true;
   }
-  // skips one value of any type
+  // skips one value of any type; nested lists and dictionaries are tracked
+  // with a depth counter, not recursion, so no nesting depth of an input can
+  // exhaust the stack
   bool skip() {
-    const int c = peek();
-    if (c == 'i') {
-      ++pos_;
-      while (pos_ < n_ && d_[pos_] != 'e') ++pos_;
-      if (pos_ >= n_) return fail("unterminated integer");
-      ++pos_;
-      return true;
-    }
-    if (c == 'l' || c == 'd') {
-      ++pos_;
-      while (ok() && peek() != 'e') {
-        if (peek() < 0) return fail("unterminated list");
-        if (!skip()) return false;
+    size_t depth = 0;
+    do {
+      const int c = peek();
+      if (c == 'i') {
+        ++pos_;
+        while (pos_ < n_ && d_[pos_] != 'e') ++pos_;
+        if (pos_ >= n_) return fail("unterminated integer");
+        ++pos_;
+      } else if (c == 'l' || c == 'd') {
+        ++pos_;
+        ++depth;
+      } else if (c == 'e' && depth > 0) {
+        ++pos_;
+        --depth;
+      } else if (c < 0) {
+        return fail("unterminated list");
+      } else {
+        const char* s;
+        size_t l;
+        if (!bytes(&s, &l)) return false;
       }
-      ++pos_;
-      return true;
-    }
-    const char* s;
-    size_t l;
-    return bytes(&s, &l);
+    } while (depth > 0);
+    return true;
   }
   size_t pos() const { return pos_; }
   size_t size() const { return n_; }

@@ -566,6 +566,13 @@ uint64_t svo_tree_device_bytes(const svo_tree* dtree) {
   return dtree && dtree->dev ? dtree->dev->bytes : 0;
 }
 
+// Pixel indices y * w + x are 32-bit in the render kernels: images hold
+// fewer than 2^32 pixels.
+static bool image_fits(uint32_t w, uint32_t h) {
+  return static_cast<uint64_t>(w) * h < (uint64_t(1) << 32);
+}
+static const char* const kImageTooLarge = "image of 2^32 or more pixels (w * h must be < 2^32)";
+
 int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hits* out,
              void* stream) {
   if (!dtree || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
@@ -577,6 +584,7 @@ int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hit
 
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream) {
+  if (!image_fits(w, h)) return fail(SVO_INVALID_ARGUMENT, kImageTooLarge);
   if (!dtree || !vp || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   const svo::Camera cam = svo::make_camera(
       svo::V3f{vp->origin[0], vp->origin[1], vp->origin[2]},
@@ -600,6 +608,7 @@ int svo_cast_at_lod(const svo_tree* dtree, const float* rays, uint32_t n,
 
 int svo_render_at_lod(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                       float viewing_distance, const svo_hits* out, void* stream) {
+  if (!image_fits(w, h)) return fail(SVO_INVALID_ARGUMENT, kImageTooLarge);
   if (!dtree || !vp || !out) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   if (!(viewing_distance > 0.0f)) return fail(SVO_INVALID_ARGUMENT, "viewing distance must be > 0");
   const svo::Camera cam = svo::make_camera(
@@ -615,6 +624,7 @@ int svo_render_at_lod(const svo_tree* dtree, const svo_viewport* vp, uint32_t w,
 
 int svo_render_rgba8(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                      uint32_t convention, uint32_t* out, void* stream) {
+  if (!image_fits(w, h)) return fail(SVO_INVALID_ARGUMENT, kImageTooLarge);
   if (!dtree || !vp) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   std::string err;
   const int st = svo::device_render_rgba8(

@@ -23,7 +23,9 @@ inline HitPtrs to_ptrs(const svo_hits& h) {
   return HitPtrs{h.hit, h.pix, h.albedo, h.voxel, h.distance, h.impact, h.normal, h.flags};
 }
 
-__device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint32_t i,
+// Result slot i of the output arrays; 64-bit, since 3 * i overflows 32 bits
+// from i = 2^32 / 3 on.
+__device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint64_t i,
                                     const CastResult& r, V3f org) {
   if (o.hit) o.hit[i] = r.hit;
   if (o.pix) o.pix[i] = r.hit ? r.pix : EMPTY_MARKER;

@@ -450,12 +450,60 @@ def alloc_hits(n: int, device="cuda", fields=HIT_FIELDS) -> dict:
     return {k: torch.empty(shapes[k], dtype=dtypes[k], device=device) for k in fields}
 
 
-def _hits_struct(bufs: dict) -> N.Hits:
+_FIELD_WIDTH = {"hit": 1, "pix": 1, "albedo": 1, "voxel": 3, "distance": 1, "impact": 3,
+                "normal": 3, "flags": 1}
+
+
+def _check_device_tensor(t, name: str, dtype, elems: int, device: bool = True) -> None:
+    """Raises ValueError unless `t` is a contiguous device tensor of `dtype`
+    holding at least `elems` elements: the kernels write through its raw
+    address and know nothing of its size. (device=False leaves out the
+    device test, for callers that make it in a second pass.)"""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if t.numel() < elems:
+        raise ValueError(f"{name}: {t.numel()} elements, at least {elems} needed")
+    if device and t.device.type != "cuda":
+        raise ValueError(f"{name}: tensor is on {t.device}, not a HIP device")
+
+
+def _hits_struct(bufs: dict, n: int) -> N.Hits:
+    """Output pointers for n rays, after checking every buffer: int32 for hit,
+    pix, albedo, voxel and flags, float32 for distance, impact and normal,
+    contiguous, on a HIP device, with n (x3 for voxel, impact and normal)
+    elements at least."""
+    torch = _torch()
+    unknown = set(bufs) - set(HIT_FIELDS)
+    if unknown:
+        raise ValueError(f"unknown output fields {sorted(unknown)}")
     h = N.Hits()
     for k in HIT_FIELDS:
         t = bufs.get(k)
+        if t is not None:
+            dt = torch.float32 if k in ("distance", "impact", "normal") else torch.int32
+            _check_device_tensor(t, k, dt, n * _FIELD_WIDTH[k], device=False)
+    for k in HIT_FIELDS:
+        t = bufs.get(k)
+        if t is not None and t.device.type != "cuda":
+            raise ValueError(f"{k}: tensor is on {t.device}, not a HIP device")
         setattr(h, k, t.data_ptr() if t is not None else None)
     return h
+
+
+def _check_rays(rays, n: int):
+    """A contiguous (>= n) x 6 float32 device tensor of origins and directions."""
+    torch = _torch()
+    _check_device_tensor(rays, "rays", torch.float32, 6 * n, device=False)
+    if rays.dim() != 2 or rays.shape[1] != 6:
+        raise ValueError(f"rays: shape {tuple(rays.shape)}, expected (n, 6)")
+    if rays.device.type != "cuda":
+        raise ValueError(f"rays: tensor is on {rays.device}, not a HIP device")
+    return rays
 
 
 def _stream_ptr(stream) -> int | None:
@@ -514,7 +562,8 @@ class DeviceTree:
         return int(r.value), int(s.value)
 
     def cast_into(self, rays, n: int, bufs: dict, stream=None) -> None:
-        hs = _hits_struct(bufs)
+        _check_rays(rays, n)
+        hs = _hits_struct(bufs, n)
         _check(N.lib().svo_cast(self._h, rays.data_ptr(), n, C.byref(hs), _stream_ptr(stream)))
 
     def cast(self, rays, fields=HIT_FIELDS, stream=None) -> dict:
@@ -527,7 +576,7 @@ class DeviceTree:
         return bufs
 
     def render_into(self, vp: Viewport, w: int, h: int, bufs: dict, stream=None) -> None:
-        hs = _hits_struct(bufs)
+        hs = _hits_struct(bufs, w * h)
         v = vp.to_c()
         _check(N.lib().svo_render(self._h, C.byref(v), w, h, C.byref(hs), _stream_ptr(stream)))
 
@@ -552,7 +601,7 @@ class DeviceTree:
         rays = rays.to(dtype=torch.float32).contiguous()
         n = rays.shape[0]
         bufs = alloc_hits(n, rays.device, fields)
-        hs = _hits_struct(bufs)
+        hs = _hits_struct(bufs, n)
         _check(N.lib().svo_cast_at_lod(self._h, rays.data_ptr(), n, float(viewing_distance),
                                        C.byref(hs), _stream_ptr(stream)))
         return bufs
@@ -561,7 +610,7 @@ class DeviceTree:
                       fields=HIT_FIELDS, device="cuda", stream=None) -> dict:
         """get_by_ray_at_lod for the primary rays of a viewport."""
         bufs = alloc_hits(w * h, device, fields)
-        hs = _hits_struct(bufs)
+        hs = _hits_struct(bufs, w * h)
         v = vp.to_c()
         _check(N.lib().svo_render_at_lod(self._h, C.byref(v), w, h, float(viewing_distance),
                                          C.byref(hs), _stream_ptr(stream)))
@@ -573,6 +622,7 @@ def _render_rgba8(tree: "DeviceTree", vp: "Viewport", w: int, h: int, convention
     torch = _torch()
     if out is None:
         out = torch.empty((h, w), dtype=torch.int32, device="cuda")
+    _check_device_tensor(out, "out", torch.int32, w * h)
     v = vp.to_c()
     _check(N.lib().svo_render_rgba8(tree._h, C.byref(v), w, h, convention, out.data_ptr(),
                                     _stream_ptr(stream)))
@@ -599,19 +649,33 @@ class Batch:
     @staticmethod
     def render_job(tree: "DeviceTree", vp: "Viewport", w: int, h: int, bufs: dict,
                    viewing_distance: float = 0.0) -> N.Job:
-        """A viewport render; with viewing_distance > 0, get_by_ray_at_lod."""
-        return N.Job(tree._h, 0, w, h, vp.to_c(), None, _hits_struct(bufs),
-                     float(viewing_distance))
+        """A viewport render; with viewing_distance > 0, get_by_ray_at_lod.
+        The job holds references to the tree and the output tensors."""
+        j = N.Job(tree._h, 0, w, h, vp.to_c(), None, _hits_struct(bufs, w * h),
+                  float(viewing_distance))
+        j._refs = (tree, dict(bufs))
+        return j
 
     @staticmethod
     def cast_job(tree: "DeviceTree", rays, bufs: dict, viewing_distance: float = 0.0) -> N.Job:
-        """A ray-array cast; with viewing_distance > 0, get_by_ray_at_lod."""
-        return N.Job(tree._h, 1, rays.shape[0], 1, N.Viewport(), rays.data_ptr(),
-                     _hits_struct(bufs), float(viewing_distance))
+        """A ray-array cast; with viewing_distance > 0, get_by_ray_at_lod.
+        The job holds references to the tree, the rays and the outputs."""
+        n = int(rays.shape[0])
+        _check_rays(rays, n)
+        j = N.Job(tree._h, 1, n, 1, N.Viewport(), rays.data_ptr(), _hits_struct(bufs, n),
+                  float(viewing_distance))
+        j._refs = (tree, rays, dict(bufs))
+        return j
 
     def run(self, jobs: list, stream=None) -> None:
+        """Launches the jobs asynchronously on `stream`. The jobs (and through
+        them their trees and tensors) stay referenced until the next run, so
+        the memory a launch in flight writes is not released under it."""
+        if len(jobs) > self.max_jobs:
+            raise ValueError(f"{len(jobs)} jobs, the batch was created for {self.max_jobs}")
         arr = (N.Job * max(1, len(jobs)))(*jobs)
         _check(N.lib().svo_batch_run(self._h, arr, len(jobs), _stream_ptr(stream)))
+        self._keep = list(jobs)
 
     @property
     def tiles(self) -> int:
