@@ -191,7 +191,9 @@ int svo_octree_load_vox(const char* path, uint32_t brick_dim, svo_octree** out);
  * kind 2: 3D value-noise terrain with worm caves and seeded brick forms
  *         (40% Empty, 10% Solid, rest Parted), built bottom-up;
  * kind 3: seeded spheres over a noise heightfield with 50/50 Parted/Solid
- *         leaf bricks, built bottom-up. */
+ *         leaf bricks, built bottom-up;
+ * kind 4: the kind 1 heightfield (same voxels) built bottom-up, for sizes
+ *         where inserting voxel by voxel takes too long. */
 int svo_scene_build(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t seed,
                     svo_octree** out);
 
@@ -250,9 +252,17 @@ int svo_tree_from_octree(svo_octree* tree, svo_tree** out);
 /* Bring a device tree up to date with edits made to the host octree since it
  * was created or last updated (OctreeGPUView::reload,
  * src/raytracing/bevy/mod.rs:56-60): only node records and bricks that
- * changed are uploaded (a full upload when the arrays must grow). Consumes the
- * octree's log of changed nodes, so one host octree feeds one device tree.
- * Ordered on `stream`; returns after the upload has completed. */
+ * changed are uploaded; arrays that must grow are moved to larger ones on the
+ * device. Consumes the octree's log of changed nodes, so one host octree
+ * feeds one device tree.
+ * Asynchronous on `stream`: the call returns once the patch is queued (it
+ * waits on the host only for the update two calls back, whose pinned staging
+ * it reuses), and the host octree may be edited again right away. Casts that
+ * must see the update run on `stream` after it or wait for an event recorded
+ * there; casts still running on other streams must have finished before the
+ * patch runs, which the caller orders (as `&mut self` does in the
+ * reference). A full re-upload (a tree created from a description, or a new
+ * brick dimension) completes before the call returns. */
 int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream);
 /* Records and brick slots written by the last create / update. */
 int svo_tree_update_stats(const svo_tree* dtree, uint32_t* records, uint32_t* slots);
@@ -404,11 +414,13 @@ int svo_current_batch_variant(void);
 /*    svo_render launch, in arrival order, on up to max_inflight streams.  */
 /*  mode 2: mode 0 with the persistent batched kernel for every launch.    */
 /* Both modes keep per-tree order: a render sees exactly the edits that     */
-/* arrived before it. Edits run on a host worker thread (host octree edit,  */
-/* then svo_tree_update on its own stream) once no earlier render of the    */
-/* tree is queued or running. Outputs of a render (hit, albedo, voxel,      */
-/* distance) stay in device memory owned by the loop until its slot is      */
-/* reused.                                                                  */
+/* arrived before it. Edits run on a pool of host threads (host octree      */
+/* edit, then an asynchronous svo_tree_update on the thread's own stream)   */
+/* once every earlier render of the tree has been launched; the patch waits */
+/* on the device for those launches, and later renders of the tree wait on  */
+/* the device for the patch. An edit completes when its patch has run.      */
+/* Outputs of a render (hit, albedo, voxel, distance) stay in device memory */
+/* owned by the loop until its slot is reused.                              */
 /* ---------------------------------------------------------------------- */
 
 typedef struct svo_serve_config {
@@ -454,8 +466,9 @@ typedef struct svo_serve_stats {
   double edit_p50_ms, edit_p99_ms;        /* edit requests only */
   double mean_jobs_per_launch;
   uint64_t edit_records, edit_bricks;     /* node records / brick slots uploaded by edits */
-  /* phases of edit requests: arrival to start on the worker (waiting for the
-   * tree to go idle), host octree edit, layout update and device upload */
+  /* phases of edit requests: arrival to start on a worker (waiting for the
+   * earlier renders of the tree to launch), host octree edit and layout
+   * update, and the device time of the patch (copy and scatter) */
   double edit_wait_p99_ms, edit_apply_p99_ms, edit_upload_p99_ms;
   uint64_t timed_out;       /* requests dropped by the deadline (latency -2) */
   uint64_t single_launches; /* mode 0: launches that held one whole request and ran as a

@@ -324,6 +324,13 @@ int hip_fail(hipError_t e, const char* what, std::string* err) {
   return e == hipErrorOutOfMemory ? SVO_OUT_OF_DEVICE_MEMORY : SVO_HIP_ERROR;
 }
 
+// The render kernels index pixels y * w + x in 32 bits.
+bool image_fits(uint32_t w, uint32_t h, std::string* err) {
+  if (static_cast<uint64_t>(w) * h < (uint64_t(1) << 32)) return true;
+  *err = "image of 2^32 or more pixels (w * h must be < 2^32)";
+  return false;
+}
+
 // Checks that every index the kernels follow stays inside the arrays.
 bool validate_desc(const svo_tree_desc& d, std::string* err) {
   if (d.node_count == 0 || d.nodes == nullptr) {
@@ -439,26 +446,36 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
     *cap = nc;
     return SVO_OK;
   };
-  uint64_t old_bytes_nodes = t->node_capacity, old_bricks = t->brick_capacity;
-  (void)old_bytes_nodes;
-  (void)old_bricks;
   int st = grow(reinterpret_cast<void**>(&t->nodes), &t->node_capacity, d.node_count,
                 sizeof(NodeRec));
   if (st != SVO_OK) return st;
-  uint64_t bcap = t->brick_capacity;
-  st = grow(reinterpret_cast<void**>(&t->brick_pix), &t->brick_capacity,
-            d.brick_count ? d.brick_count : 1, n3 * sizeof(uint32_t));
-  if (st != SVO_OK) return st;
-  if (t->brick_capacity != bcap || t->brick_occ == nullptr) {
-    if (t->brick_occ) {
+  // The brick arrays are compared in bytes, not slots: the description may
+  // carry another brick dimension than the resident tree (a slot of 16^3
+  // values is 64 times one of 4^3), and both arrays must hold its bricks.
+  const uint64_t slots = d.brick_count ? d.brick_count : 1;
+  const uint64_t pix_need = slots * n3 * sizeof(uint32_t);
+  const uint64_t occ_need = slots * words * sizeof(uint32_t);
+  auto grow_bytes = [&](uint32_t** p, uint64_t* cap_bytes, uint64_t need,
+                        uint64_t slot_bytes) -> int {
+    if (need <= *cap_bytes && *p) return SVO_OK;
+    const uint64_t nb = grown_capacity(slots) * slot_bytes;
+    void* np = nullptr;
+    hipError_t ge = hipMalloc(&np, nb);
+    if (ge != hipSuccess) return hip_fail(ge, "hipMalloc(bricks)", err);
+    if (*p) {
       (void)hipStreamSynchronize(s);
-      (void)hipFree(t->brick_occ);
-      t->brick_occ = nullptr;
+      (void)hipFree(*p);
     }
-    hipError_t ge = hipMalloc(reinterpret_cast<void**>(&t->brick_occ),
-                              t->brick_capacity * words * sizeof(uint32_t));
-    if (ge != hipSuccess) return hip_fail(ge, "hipMalloc", err);
-  }
+    *p = static_cast<uint32_t*>(np);
+    *cap_bytes = nb;
+    return SVO_OK;
+  };
+  st = grow_bytes(&t->brick_pix, &t->pix_bytes, pix_need, n3 * sizeof(uint32_t));
+  if (st != SVO_OK) return st;
+  st = grow_bytes(&t->brick_occ, &t->occ_bytes, occ_need, words * sizeof(uint32_t));
+  if (st != SVO_OK) return st;
+  t->brick_capacity = std::min(t->pix_bytes / (n3 * sizeof(uint32_t)),
+                               t->occ_bytes / (words * sizeof(uint32_t)));
   st = grow(reinterpret_cast<void**>(&t->colors), &t->color_capacity,
             d.color_count ? d.color_count : 1, sizeof(uint32_t));
   if (st != SVO_OK) return st;
@@ -487,9 +504,8 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   t->brick_count = d.brick_count;
   t->color_count = d.color_count;
   t->top_level_nodes = d.top_level_nodes;
-  t->bytes = t->node_capacity * sizeof(NodeRec) + t->brick_capacity * n3 * sizeof(uint32_t) +
-             t->brick_capacity * words * sizeof(uint32_t) + t->color_capacity * sizeof(uint32_t) +
-             64 * 8 * sizeof(uint64_t);
+  t->bytes = t->node_capacity * sizeof(NodeRec) + t->pix_bytes + t->occ_bytes +
+             t->color_capacity * sizeof(uint32_t) + 64 * 8 * sizeof(uint64_t);
   t->view.nodes = t->nodes;
   t->view.brick_pix = t->brick_pix;
   t->view.brick_occ = t->brick_occ;
@@ -585,21 +601,25 @@ int grow_resident(DeviceTree* t, const svo_tree_desc& d, hipStream_t s, std::str
                 regrown_capacity(d.node_count), t->node_count * sizeof(NodeRec), sizeof(NodeRec), s,
                 err);
   if (st == SVO_OK && d.brick_count > t->brick_capacity) {
+    // (same brick dimension as the resident tree: device_tree_patch sends a
+    // change of dimension to device_tree_upload)
     const uint64_t nc = regrown_capacity(d.brick_count);
-    uint64_t occ_cap = t->brick_capacity;
-    st = regrow(t, reinterpret_cast<void**>(&t->brick_pix), &t->brick_capacity, nc,
+    uint64_t pix_cap = t->pix_bytes / (n3 * 4), occ_cap = t->occ_bytes / (words * 4);
+    st = regrow(t, reinterpret_cast<void**>(&t->brick_pix), &pix_cap, nc,
                 t->brick_count * n3 * 4, n3 * 4, s, err);
+    if (st == SVO_OK) t->pix_bytes = pix_cap * n3 * 4;
     if (st == SVO_OK)
       st = regrow(t, reinterpret_cast<void**>(&t->brick_occ), &occ_cap, nc,
                   t->brick_count * words * 4, words * 4, s, err);
+    if (st == SVO_OK) t->occ_bytes = occ_cap * words * 4;
+    t->brick_capacity = std::min(t->pix_bytes / (n3 * 4), t->occ_bytes / (words * 4));
   }
   if (st == SVO_OK && d.color_count > t->color_capacity)
     st = regrow(t, reinterpret_cast<void**>(&t->colors), &t->color_capacity,
                 grown_color_capacity(d.color_count), t->color_count * 4ull, 4, s, err);
   if (st != SVO_OK) return st;
-  t->bytes = t->node_capacity * sizeof(NodeRec) + t->brick_capacity * n3 * sizeof(uint32_t) +
-             t->brick_capacity * words * sizeof(uint32_t) + t->color_capacity * sizeof(uint32_t) +
-             64 * 8 * sizeof(uint64_t);
+  t->bytes = t->node_capacity * sizeof(NodeRec) + t->pix_bytes + t->occ_bytes +
+             t->color_capacity * sizeof(uint32_t) + 64 * 8 * sizeof(uint64_t);
   t->view.nodes = t->nodes;
   t->view.brick_pix = t->brick_pix;
   t->view.brick_occ = t->brick_occ;
@@ -609,31 +629,65 @@ int grow_resident(DeviceTree* t, const svo_tree_desc& d, hipStream_t s, std::str
 
 }  // namespace
 
-int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err) {
-  if (bytes <= t->staging_bytes && bytes <= t->host_staging_bytes) return SVO_OK;
-  hipError_t e = hipSetDevice(t->device);
-  if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
+namespace {
+
+// Makes staging set `ps` hold `bytes`; the set is not in use (its last patch
+// has finished).
+int reserve_set(DeviceTree* t, DeviceTree::PatchStaging& ps, size_t bytes, std::string* err) {
+  hipError_t e;
+  if (!ps.done) {
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "hipEventCreate(staging)", err);
+    ps.done = ev;
+  }
+  if (bytes <= ps.dev_bytes && bytes <= ps.host_bytes) return SVO_OK;
   const size_t cap = std::max<size_t>(bytes + bytes / 2, size_t(8) << 20);
-  if (bytes > t->staging_bytes) {
+  if (bytes > ps.dev_bytes) {
     // a smaller buffer is retired, not freed: freeing waits for the whole
     // device; growth is geometric, so few buffers are ever retired
     void* grown = nullptr;
     e = hipMalloc(&grown, cap);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(staging)", err);
-    if (t->staging) t->retired.push_back(t->staging);
-    t->staging = grown;
-    t->staging_bytes = cap;
-  }
-  if (bytes > t->host_staging_bytes) {
-    if (t->host_staging) {
-      // the previous patch synchronized its stream, so nothing reads it
-      (void)hipHostFree(t->host_staging);
+    if (ps.dev) {
+      t->retired.push_back(ps.dev);
+      t->retired_bytes += ps.dev_bytes;
     }
-    t->host_staging = nullptr;
-    t->host_staging_bytes = 0;
-    e = hipHostMalloc(&t->host_staging, cap, hipHostMallocDefault);
+    ps.dev = grown;
+    ps.dev_bytes = cap;
+  }
+  if (bytes > ps.host_bytes) {
+    if (ps.host) (void)hipHostFree(ps.host);  // the set's last copy has finished
+    ps.host = nullptr;
+    ps.host_bytes = 0;
+    e = hipHostMalloc(&ps.host, cap, hipHostMallocDefault);
     if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)", err);
-    t->host_staging_bytes = cap;
+    ps.host_bytes = cap;
+  }
+  return SVO_OK;
+}
+
+// Waits (on the host) until staging set `ps` is no longer read by a patch.
+int drain_set(DeviceTree::PatchStaging& ps, std::string* err) {
+  if (!ps.pending) return SVO_OK;
+  const hipError_t e = hipEventSynchronize(static_cast<hipEvent_t>(ps.done));
+  if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize(staging)", err);
+  ps.pending = false;
+  return SVO_OK;
+}
+
+}  // namespace
+
+int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err) {
+  hipError_t e = hipSetDevice(t->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
+  for (DeviceTree::PatchStaging& ps : t->staging) {
+    if (bytes > ps.dev_bytes || bytes > ps.host_bytes) {
+      const int st = drain_set(ps, err);
+      if (st != SVO_OK) return st;
+    }
+    const int st = reserve_set(t, ps, bytes, err);
+    if (st != SVO_OK) return st;
   }
   return SVO_OK;
 }
@@ -674,9 +728,13 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   const size_t o_slots = align16(o_bw + slots.size() * static_cast<size_t>(n3 + words) * 4);
   const size_t o_colors = align16(o_slots + slots.size() * 4);
   const size_t bytes = align16(o_colors + new_colors * 4u);
-  const int sst = device_tree_reserve_staging(t, bytes, err);
+  // the set the patch two calls back used: its copy and scatter must have
+  // finished before the host refills it
+  DeviceTree::PatchStaging& ps = t->staging[t->next_staging];
+  int sst = drain_set(ps, err);
+  if (sst == SVO_OK) sst = reserve_set(t, ps, bytes, err);
   if (sst != SVO_OK) return sst;
-  char* h = static_cast<char*>(t->host_staging);
+  char* h = static_cast<char*>(ps.host);
   for (size_t i = 0; i < records.size(); ++i)
     std::memcpy(h + i * sizeof(NodeRec), &d.nodes[records[i]], sizeof(NodeRec));
   if (!records.empty()) std::memcpy(h + o_ridx, records.data(), records.size() * 4);
@@ -688,7 +746,7 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   }
   if (!slots.empty()) std::memcpy(h + o_slots, slots.data(), slots.size() * 4);
   if (new_colors) std::memcpy(h + o_colors, d.colors + colors_from, new_colors * 4u);
-  char* base = static_cast<char*>(t->staging);
+  char* base = static_cast<char*>(ps.dev);
   if (bytes > 0) {
     e = hipMemcpyAsync(base, h, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(patch)", err);
@@ -716,9 +774,11 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
                        hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(colors)", err);
   }
-  // the pinned staging is reused by the next patch
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize", err);
+  // no wait here: the set stays pending until the stream passes this event
+  e = hipEventRecord(static_cast<hipEvent_t>(ps.done), s);
+  if (e != hipSuccess) return hip_fail(e, "hipEventRecord(staging)", err);
+  ps.pending = true;
+  t->next_staging ^= 1u;
   t->node_count = d.node_count;
   t->brick_count = d.brick_count;
   t->color_count = d.color_count;
@@ -732,14 +792,17 @@ void device_tree_destroy(DeviceTree* t) {
   if (!t) return;
   if (t->device >= 0) (void)hipSetDevice(t->device);
   (void)hipDeviceSynchronize();
-  if (t->host_staging) (void)hipHostFree(t->host_staging);
+  for (DeviceTree::PatchStaging& ps : t->staging) {
+    if (ps.host) (void)hipHostFree(ps.host);
+    if (ps.dev) (void)hipFree(ps.dev);
+    if (ps.done) (void)hipEventDestroy(static_cast<hipEvent_t>(ps.done));
+  }
   for (void* p : t->retired) (void)hipFree(p);
   if (t->nodes) (void)hipFree(t->nodes);
   if (t->brick_pix) (void)hipFree(t->brick_pix);
   if (t->brick_occ) (void)hipFree(t->brick_occ);
   if (t->colors) (void)hipFree(t->colors);
   if (t->ray_lut) (void)hipFree(t->ray_lut);
-  if (t->staging) (void)hipFree(t->staging);
   delete t;
 }
 
@@ -752,8 +815,10 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
+  // n < 2^32, so the grid (at most 2^24 workgroups) is computed without
+  // wrapping in 64 bits
   const uint32_t block = 256;
-  const uint32_t grid = (n + block - 1) / block;
+  const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(n) + block - 1) / block);
   hipLaunchKernelGGL(cast_kernel, dim3(grid), dim3(block), 0, s, t.view, rays, n, to_ptrs(out));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "cast_kernel launch", err);
@@ -770,7 +835,7 @@ int device_cast_lod(const DeviceTree& t, const float* rays, uint32_t n, float vi
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
-  const uint32_t grid = (n + 255u) / 256u;
+  const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(n) + 255u) / 256u);
   hipLaunchKernelGGL(cast_lod_kernel, dim3(grid), dim3(256), 0, s, t.view, rays, n,
                      viewing_distance, to_ptrs(out));
   hipError_t e = hipGetLastError();
@@ -783,6 +848,7 @@ int device_render_lod(const DeviceTree& t, const Camera& cam, uint32_t w, uint32
                       float viewing_distance, const svo_hits& out, void* stream,
                       std::string* err) {
   if (w == 0 || h == 0) return SVO_OK;
+  if (!image_fits(w, h, err)) return SVO_INVALID_ARGUMENT;
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
   const dim3 grid((w + 15u) / 16u, (h + 15u) / 16u);
@@ -797,6 +863,7 @@ int device_render_lod(const DeviceTree& t, const Camera& cam, uint32_t w, uint32
 int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
                   const svo_hits& out, void* stream, std::string* err) {
   if (w == 0 || h == 0) return SVO_OK;
+  if (!image_fits(w, h, err)) return SVO_INVALID_ARGUMENT;
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
   const RenderVariant& v = kRenderVariants[g_render_variant];
@@ -827,6 +894,7 @@ int device_render_rgba8(const DeviceTree& t, V3f origin, V3f dir, float glass_w,
     return SVO_INVALID_ARGUMENT;
   }
   if (w == 0 || h == 0) return SVO_OK;
+  if (!image_fits(w, h, err)) return SVO_INVALID_ARGUMENT;
   if (!out) {
     *err = "output is NULL";
     return SVO_INVALID_ARGUMENT;

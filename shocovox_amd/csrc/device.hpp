@@ -18,17 +18,32 @@ struct DeviceTree {
   uint32_t color_count = 0;
   uint32_t top_level_nodes = 0;
   uint64_t node_capacity = 0;   // records allocated
-  uint64_t brick_capacity = 0;  // brick slots allocated
+  uint64_t brick_capacity = 0;  // brick slots that fit both brick arrays at the current dim
   uint64_t color_capacity = 0;
+  // bytes allocated for brick_pix and brick_occ: a re-upload may change the
+  // brick dimension, and with it the bytes of a slot, so the arrays are
+  // sized in bytes and brick_capacity is derived from them
+  uint64_t pix_bytes = 0;
+  uint64_t occ_bytes = 0;
   NodeRec* nodes = nullptr;
   uint32_t* brick_pix = nullptr;
   uint32_t* brick_occ = nullptr;
   uint32_t* colors = nullptr;
   uint64_t* ray_lut = nullptr;
-  void* staging = nullptr;  // device staging of incremental patches
-  uint64_t staging_bytes = 0;
-  void* host_staging = nullptr;  // pinned host staging of incremental patches
-  uint64_t host_staging_bytes = 0;
+  // Staging of incremental patches, two sets used alternately: a patch fills
+  // one (pinned host buffer, device buffer) while the copy and scatter of the
+  // previous patch may still be reading the other; `done` marks the end of
+  // the last patch that used a set.
+  struct PatchStaging {
+    void* host = nullptr;
+    uint64_t host_bytes = 0;
+    void* dev = nullptr;
+    uint64_t dev_bytes = 0;
+    void* done = nullptr;  // hipEvent_t
+    bool pending = false;
+  };
+  PatchStaging staging[2];
+  uint32_t next_staging = 0;
   std::vector<void*> retired;    // outgrown device buffers, freed with the tree
   uint64_t retired_bytes = 0;
   uint32_t step_limit = 0;       // lowered iteration bound per ray (0: MAX_TRAVERSAL_STEPS)
@@ -45,14 +60,17 @@ void device_tree_destroy(DeviceTree* t);
 // Device bytes a fresh device_tree_create of `desc` allocates (capacities
 // include the growth headroom of uploads).
 uint64_t device_tree_bytes_for(const svo_tree_desc& desc);
-// Writes the given records and brick slots of a layout into an existing tree.
-// Arrays whose capacity `desc` (the layout's full description) outgrows are
-// reallocated larger and their live contents copied device to device on
-// `stream` first, so a patch never re-sends the whole tree from the host.
-// Returns 1 (asking for a full upload) only when the brick dimension differs.
-// Makes the patch staging (pinned host and device) hold at least `bytes`
+// Makes both patch stagings (pinned host and device) hold at least `bytes`
 // (at least 8 MiB), so the first edits of a tree allocate nothing.
 int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err);
+// Writes the given records and brick slots of a layout into an existing tree,
+// asynchronously on `stream`: the call returns once the copy and scatter are
+// queued (it waits on the host only for the patch two calls back, whose
+// staging it reuses). Arrays whose capacity `desc` (the layout's full
+// description) outgrows are reallocated larger and their live contents
+// copied device to device on `stream` first, so a patch never re-sends the
+// whole tree from the host. Returns 1 (asking for a full upload) only when
+// the brick dimension differs.
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& desc,
                       const std::vector<uint32_t>& records, const std::vector<uint32_t>& slots,
                       uint32_t colors_from, void* stream, std::string* err);

@@ -1,7 +1,16 @@
 // Serving loop: seeded Poisson arrivals of ray-cast requests over resident
 // trees, served by continuous batching (mode 0) or by one plain launch per
-// request (mode 1), with sparse voxel edits applied in place on a worker
-// thread in per-tree arrival order.
+// request (mode 1), with sparse voxel edits applied in place in per-tree
+// arrival order.
+//
+// Edits run on a pool of host threads (host octree edit and layout update,
+// then an asynchronous patch on the thread's own stream). The host never
+// waits for the device to order them: an edit starts once every earlier
+// render of its tree has been launched, its patch waits on the device for
+// the launches still running them (events recorded on their streams) and
+// for the tree's previous patch, and launches that render the tree after the
+// edit wait on the device for its patch. Every wait refers to an event
+// recorded earlier on the host, so the waits cannot form a cycle.
 //
 // Continuous batching here is iteration-level scheduling: every batched
 // launch is short (at most max_batch_rays rays), and at each launch
@@ -95,6 +104,8 @@ constexpr double kDrainSeconds = 120.0;
 constexpr uint64_t kMinShareTiles = 64;
 // pause between polls of running launches and edits when nothing changed
 constexpr double kPollSeconds = 20e-6;
+// host threads that apply edits (each tree's edits stay in order on one at a time)
+constexpr uint32_t kEditWorkers = 4;
 
 int check_config(const svo_serve_config* cfg, uint32_t n_trees) {
   if (!cfg) return serve_fail(SVO_INVALID_ARGUMENT, "config is NULL");
@@ -281,16 +292,31 @@ struct Slot {
 };
 
 struct EditTimes {
-  double start = 0, applied = 0, uploaded = 0;
+  double start = 0, applied = 0;  // host clock: edit started, host octree and layout edited
+  double upload_ms = 0;           // device time of the patch (its copy and scatter)
 };
 
-struct EditWorker {
-  std::thread th;
+// One edit handed to the workers. The patch waits on the device for `waits`
+// (events recorded on the streams of launches that hold earlier renders of
+// the tree) and, after an earlier edit of the tree, for that edit's patch;
+// `start` and `done` bracket the patch on the worker's stream.
+struct EditJob {
+  uint32_t req = 0;
+  bool after_edit = false;
+  std::vector<hipEvent_t> waits;
+  hipEvent_t start = nullptr, done = nullptr;
+};
+
+// Edits run on a pool of host threads, each with its own stream: edits of
+// different trees proceed in parallel, edits of one tree never overlap (the
+// loop hands out the next edit of a tree only after the previous one's host
+// part has finished).
+struct EditPool {
+  std::vector<std::thread> threads;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<uint32_t> todo;
-  std::vector<std::pair<uint32_t, double>> finished;
-  std::vector<EditTimes> times;  // per request of the plan (edits only)
+  std::deque<EditJob> todo;
+  std::vector<std::pair<EditJob, EditTimes>> finished;  // host part done, patch queued
   bool stop = false;
   int status = SVO_OK;
   std::string error;
@@ -308,9 +334,26 @@ struct Session {
   hipEvent_t ref = nullptr;
   std::vector<Slot> slots;
   unsigned long long* d_digests = nullptr;
-  EditWorker worker;
+  EditPool edits;
+  // per tree: recorded after its last patch; launches that render the tree
+  // after an edit wait for it on the device
+  std::vector<hipEvent_t> tree_ready;
+  std::vector<uint8_t> has_ready;
+  std::vector<hipEvent_t> events;  // every event the session created
+  std::vector<hipEvent_t> free_events;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+  hipEvent_t take_event() {
+    if (!free_events.empty()) {
+      hipEvent_t e = free_events.back();
+      free_events.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    events.push_back(e);
+    return e;
+  }
 };
 
 uint64_t tiles_of(const svo_serve_request& q) {
@@ -323,7 +366,7 @@ int hip_err(hipError_t e, const char* what) {
 }
 
 void edit_worker_main(Session* s) {
-  EditWorker& w = s->worker;
+  EditPool& w = s->edits;
   (void)hipSetDevice(s->device);
   hipStream_t stream = nullptr;
   if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
@@ -332,40 +375,58 @@ void edit_worker_main(Session* s) {
     w.error = "edit stream creation failed";
   }
   for (;;) {
-    uint32_t r;
+    EditJob job;
     {
       std::unique_lock<std::mutex> g(w.mu);
       w.cv.wait(g, [&] { return w.stop || !w.todo.empty(); });
       if (w.todo.empty()) break;
-      r = w.todo.front();
+      job = std::move(w.todo.front());
       w.todo.pop_front();
     }
-    const svo_serve_request& q = s->plan[r];
+    const svo_serve_request& q = s->plan[job.req];
     int st = SVO_HIP_ERROR;
     std::string err = "edit stream missing";
     uint32_t recs = 0, bricks = 0;
     EditTimes et;
     et.start = s->now();
     if (stream) {
+      // device order: after the launches that hold earlier renders of the
+      // tree, and after the tree's previous patch
+      hipError_t e = hipSuccess;
+      for (hipEvent_t ev : job.waits)
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev, 0);
+      if (e == hipSuccess && job.after_edit)
+        e = hipStreamWaitEvent(stream, s->tree_ready[q.tree], 0);
       st = apply_edit(s->trees[q.tree]->tree, q);
       et.applied = s->now();
-      if (st == SVO_OK) st = svo_tree_update(s->dtrees[q.tree], s->trees[q.tree], stream);
-      if (st == SVO_OK) (void)svo_tree_update_stats(s->dtrees[q.tree], &recs, &bricks);
-      err = st == SVO_OK ? "" : svo_last_error();
+      if (e == hipSuccess) e = hipEventRecord(job.start, stream);
+      if (e != hipSuccess) {
+        st = SVO_HIP_ERROR;
+        err = std::string("edit ordering: ") + hipGetErrorString(e);
+      } else {
+        if (st == SVO_OK) st = svo_tree_update(s->dtrees[q.tree], s->trees[q.tree], stream);
+        if (st == SVO_OK) (void)svo_tree_update_stats(s->dtrees[q.tree], &recs, &bricks);
+        err = st == SVO_OK ? "" : svo_last_error();
+        if (st == SVO_OK && (hipEventRecord(job.done, stream) != hipSuccess ||
+                             hipEventRecord(s->tree_ready[q.tree], stream) != hipSuccess)) {
+          st = SVO_HIP_ERROR;
+          err = "edit event record failed";
+        }
+      }
     }
-    const double t = s->now();
-    et.uploaded = t;
     std::lock_guard<std::mutex> g(w.mu);
-    w.times[r] = et;
     if (st != SVO_OK && w.status == SVO_OK) {
       w.status = st;
       w.error = "edit failed: " + err;
     }
     w.records += recs;
     w.bricks += bricks;
-    w.finished.emplace_back(r, t);
+    w.finished.emplace_back(std::move(job), et);
   }
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
 }
 
 void free_slot(Slot& sl) {
@@ -418,6 +479,16 @@ int launch_slot(Session& s, Slot& sl, bool* single) {
   const bool batched = s.cfg.mode == 2 || (s.cfg.mode == 0 && !whole);
   *single = s.cfg.mode == 0 && whole;
   std::string err;
+  // renders of a tree that has been edited wait, on the device, for its
+  // last patch (the host already waited for the edit's host part)
+  for (size_t i = 0; i < sl.chunks.size(); ++i) {
+    const uint32_t tr = s.plan[sl.chunks[i].req].tree;
+    bool seen = false;
+    for (size_t k = 0; k < i && !seen; ++k) seen = s.plan[sl.chunks[k].req].tree == tr;
+    if (seen || !s.has_ready[tr]) continue;
+    const hipError_t e = hipStreamWaitEvent(sl.stream, s.tree_ready[tr], 0);
+    if (e != hipSuccess) return hip_err(e, "hipStreamWaitEvent(tree)");
+  }
   if (batched) {
     std::vector<BatchJobIn> jobs(sl.chunks.size());
     for (size_t i = 0; i < sl.chunks.size(); ++i) {
@@ -536,8 +607,12 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   std::vector<uint8_t> rejected(n, 0);
   std::vector<uint64_t> next_tile(n, 0);
   std::vector<uint32_t> outstanding(n, 0);
-  std::vector<uint32_t> inflight(s.n_trees, 0);  // chunks launched, not finished
-  std::vector<uint8_t> editing(s.n_trees, 0);
+  std::vector<uint8_t> editing(s.n_trees, 0);  // host part of an edit in progress
+  struct PendingEdit {
+    uint32_t req;
+    hipEvent_t start, done;
+  };
+  std::vector<PendingEdit> pending_edits;  // patches queued on the device
   std::vector<uint8_t> blocked(s.n_trees, 0), seen_render(s.n_trees, 0);
   std::vector<uint8_t> started_edit(n, 0), dispatched(n, 0), timed_out(n, 0);
   const double deadline = cfg.deadline_ms * 1e-3;
@@ -547,23 +622,36 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   size_t next_arrival = 0, done = 0;
   uint64_t launches = 0, chunks_launched = 0, rays_done = 0, single_launches = 0;
 
-  // every tree that an edit of the plan touches gets its patch staging now,
-  // so no edit allocates pinned or device memory inside the session
+  // every tree that an edit of the plan touches gets its patch stagings and
+  // its ready event now, so no edit allocates pinned or device memory inside
+  // the session
+  uint32_t edited_trees = 0;
+  s.tree_ready.assign(s.n_trees, nullptr);
+  s.has_ready.assign(s.n_trees, 0);
   {
     std::vector<uint8_t> edited(s.n_trees, 0);
     for (const svo_serve_request& q : s.plan)
       if (q.kind == 1) edited[q.tree] = 1;
     for (uint32_t i = 0; i < s.n_trees; ++i) {
       if (!edited[i]) continue;
+      ++edited_trees;
       std::string err;
       if (device_tree_reserve_staging(s.dtrees[i]->dev, 1, &err) != SVO_OK)
         return serve_fail(SVO_HIP_ERROR, "edit staging: " + err);
+      hipError_t e = hipEventCreateWithFlags(&s.tree_ready[i], hipEventDisableTiming);
+      if (e != hipSuccess) return hip_err(e, "hipEventCreate(tree)");
     }
   }
   hipError_t e = hipEventCreate(&s.ref);
   if (e != hipSuccess) return hip_err(e, "hipEventCreate");
-  s.worker.times.assign(n, EditTimes{});
-  s.worker.th = std::thread(edit_worker_main, &s);
+  for (uint32_t i = 0; i < 64; ++i) {
+    hipEvent_t ev = s.take_event();
+    if (!ev) return serve_fail(SVO_HIP_ERROR, "event creation failed");
+    s.free_events.push_back(ev);
+  }
+  std::vector<EditTimes> edit_times(n);
+  const uint32_t workers = std::max(1u, std::min(kEditWorkers, edited_trees));
+  for (uint32_t i = 0; i < workers; ++i) s.edits.threads.emplace_back(edit_worker_main, &s);
   e = hipEventRecord(s.ref, nullptr);
   if (e == hipSuccess) e = hipEventSynchronize(s.ref);
   if (e != hipSuccess) return hip_err(e, "reference event");
@@ -605,7 +693,6 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
       const double tf = e == hipSuccess ? ms * 1e-3 : s.now();
       for (const Chunk& c : sl.chunks) {
         const svo_serve_request& q = s.plan[c.req];
-        --inflight[q.tree];
         // chunks of one request may finish out of order on different streams
         finish[c.req] = std::max(finish[c.req], tf);
         if (--outstanding[c.req] == 0 && next_tile[c.req] == tiles_of(q)) {
@@ -617,17 +704,46 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
       sl.busy = false;
       changed = true;
     }
-    // finished edits
+    // edits whose host part finished: renders of the tree may launch again
+    // (they wait for its patch on the device); the edit itself completes
+    // when its patch does
     {
-      std::lock_guard<std::mutex> g(s.worker.mu);
-      if (s.worker.status != SVO_OK) status = serve_fail(s.worker.status, s.worker.error);
-      for (const auto& f : s.worker.finished) {
-        editing[s.plan[f.first].tree] = 0;
-        finish[f.first] = f.second;
-        ++done;
+      std::lock_guard<std::mutex> g(s.edits.mu);
+      if (s.edits.status != SVO_OK) status = serve_fail(s.edits.status, s.edits.error);
+      for (auto& f : s.edits.finished) {
+        const uint32_t tr = s.plan[f.first.req].tree;
+        editing[tr] = 0;
+        s.has_ready[tr] = 1;
+        edit_times[f.first.req] = f.second;
+        for (hipEvent_t ev : f.first.waits) s.free_events.push_back(ev);
+        pending_edits.push_back(PendingEdit{f.first.req, f.first.start, f.first.done});
         changed = true;
       }
-      s.worker.finished.clear();
+      s.edits.finished.clear();
+    }
+    if (status != SVO_OK) break;
+    for (size_t i = 0; i < pending_edits.size();) {
+      PendingEdit& pe = pending_edits[i];
+      e = hipEventQuery(pe.done);
+      if (e == hipErrorNotReady) {
+        ++i;
+        continue;
+      }
+      if (e != hipSuccess) {
+        status = hip_err(e, "edit patch failed");
+        break;
+      }
+      float ms = 0.0f, up = 0.0f;
+      e = hipEventElapsedTime(&ms, s.ref, pe.done);
+      finish[pe.req] = e == hipSuccess ? ms * 1e-3 : s.now();
+      if (hipEventElapsedTime(&up, pe.start, pe.done) == hipSuccess)
+        edit_times[pe.req].upload_ms = up;
+      s.free_events.push_back(pe.start);
+      s.free_events.push_back(pe.done);
+      ++done;
+      changed = true;
+      pending_edits[i] = pending_edits.back();
+      pending_edits.pop_back();
     }
     if (status != SVO_OK) break;
 
@@ -662,14 +778,35 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
         }
         if (q.kind == 1) {
           blocked[tr] = 1;
-          if (!seen_render[tr] && inflight[tr] == 0) {
+          // an edit starts once every earlier render of its tree has been
+          // launched; the launches still running are waited for on the device
+          if (!seen_render[tr]) {
+            EditJob job;
+            job.req = r;
+            job.after_edit = s.has_ready[tr] != 0;
+            job.start = s.take_event();
+            job.done = s.take_event();
+            bool ok = job.start && job.done;
+            for (Slot& sl : s.slots) {
+              if (!ok || !sl.busy) continue;
+              bool holds = false;
+              for (const Chunk& c : sl.chunks) holds = holds || s.plan[c.req].tree == tr;
+              if (!holds) continue;
+              hipEvent_t ev = s.take_event();
+              ok = ev && hipEventRecord(ev, sl.stream) == hipSuccess;
+              if (ev) job.waits.push_back(ev);
+            }
+            if (!ok) {
+              status = serve_fail(SVO_HIP_ERROR, "edit ordering events");
+              break;
+            }
             editing[tr] = 1;
             started_edit[r] = 1;
             {
-              std::lock_guard<std::mutex> g(s.worker.mu);
-              s.worker.todo.push_back(r);
+              std::lock_guard<std::mutex> g(s.edits.mu);
+              s.edits.todo.push_back(std::move(job));
             }
-            s.worker.cv.notify_one();
+            s.edits.cv.notify_one();
           }
           continue;
         }
@@ -712,7 +849,6 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
           next_tile[r] += take[i];
           used += take[i];
           ++outstanding[r];
-          ++inflight[s.plan[r].tree];
           if (next_tile[r] == tiles_of(s.plan[r])) dispatched[r] = 1;
         }
       }
@@ -742,7 +878,7 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
     const bool progress =
         done != done_before || next_arrival != arrivals_before || launches != launches_before;
     if (!progress) {
-      bool edits_running = false;
+      bool edits_running = !pending_edits.empty();
       for (uint8_t v : editing) edits_running = edits_running || v;
       const double wait = next_arrival < n ? s.plan[next_arrival].arrival_s - s.now() : 1.0;
       if (!busy && !edits_running && !changed && wait > 2e-4)
@@ -753,11 +889,12 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
   }
 
   {
-    std::lock_guard<std::mutex> g(s.worker.mu);
-    s.worker.stop = true;
+    std::lock_guard<std::mutex> g(s.edits.mu);
+    s.edits.stop = true;
   }
-  s.worker.cv.notify_all();
-  s.worker.th.join();
+  s.edits.cv.notify_all();
+  for (std::thread& th : s.edits.threads) th.join();
+  s.edits.threads.clear();
   for (Slot& sl : s.slots)
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
   if (status != SVO_OK) return status;
@@ -820,15 +957,15 @@ int run_session(Session& s, svo_serve_stats* stats, float* latencies, uint64_t* 
     o.edit_p99_ms = percentile(edi, 0.99);
     o.mean_jobs_per_launch =
         launches ? static_cast<double>(chunks_launched) / static_cast<double>(launches) : 0.0;
-    o.edit_records = s.worker.records;
-    o.edit_bricks = s.worker.bricks;
+    o.edit_records = s.edits.records;
+    o.edit_bricks = s.edits.bricks;
     std::vector<double> wait, apply, upload;
     for (size_t i = 0; i < n; ++i) {
       if (s.plan[i].kind != 1 || rejected[i] || timed_out[i]) continue;
-      const EditTimes& et = s.worker.times[i];
+      const EditTimes& et = edit_times[i];
       wait.push_back((et.start - s.plan[i].arrival_s) * 1e3);
       apply.push_back((et.applied - et.start) * 1e3);
-      upload.push_back((et.uploaded - et.applied) * 1e3);
+      upload.push_back(et.upload_ms);
     }
     o.edit_wait_p99_ms = percentile(wait, 0.99);
     o.edit_apply_p99_ms = percentile(apply, 0.99);
@@ -890,15 +1027,19 @@ int svo_serve_run(svo_octree* const* trees, svo_tree* const* dtrees, uint32_t n_
   s.device = device;
   s.plan = svo::make_plan(*cfg, sizes.data(), n_trees);
   st = svo::run_session(s, stats, latencies_ms, digests, cap);
-  if (s.worker.th.joinable()) {
+  if (!s.edits.threads.empty()) {
     {
-      std::lock_guard<std::mutex> g(s.worker.mu);
-      s.worker.stop = true;
+      std::lock_guard<std::mutex> g(s.edits.mu);
+      s.edits.stop = true;
     }
-    s.worker.cv.notify_all();
-    s.worker.th.join();
+    s.edits.cv.notify_all();
+    for (std::thread& th : s.edits.threads) th.join();
   }
   for (svo::Slot& sl : s.slots) svo::free_slot(sl);
+  (void)hipDeviceSynchronize();  // patches of a failed session may still be queued
+  for (hipEvent_t ev : s.events) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : s.tree_ready)
+    if (ev) (void)hipEventDestroy(ev);
   if (s.d_digests) (void)hipFree(s.d_digests);
   if (s.ref) (void)hipEventDestroy(s.ref);
   return st;
