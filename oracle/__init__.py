@@ -18,7 +18,7 @@ LIB_PATH = HERE / "libsvo_oracle.so"
 
 HIT_DTYPE = np.dtype([("hit", "<u4"), ("pix", "<u4"), ("albedo", "<u4"), ("flags", "<u4"),
                       ("voxel", "<u4", (3,)), ("distance", "<f4"), ("impact", "<f4", (3,)),
-                      ("normal", "<f4", (3,))])
+                      ("normal", "<f4", (3,)), ("boundary", "<u4")])
 
 
 class _Dump(C.Structure):
@@ -35,7 +35,7 @@ class _Dump(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "stack_iterations", "child_reads",
                                           "brick_reads", "voxel_tests", "data_palette_reads",
-                                          "hits")]
+                                          "hits", "boundary_hits")]
 
 
 _lib = None

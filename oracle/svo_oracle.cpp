@@ -24,6 +24,7 @@
 //     plus invariants (hit voxel is non-empty under get(); a miss never has
 //     a non-empty voxel on a brute-force f64 march away from boundaries);
 //   * get: the reference's insert/clear known-answer tests.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -362,7 +363,7 @@ struct Node {
 
 struct Counters {
   uint64_t stack_iterations = 0, child_reads = 0, brick_reads = 0, voxel_tests = 0,
-           data_palette_reads = 0, hits = 0;
+           data_palette_reads = 0, hits = 0, boundary_hits = 0;
 };
 
 struct Octree {
@@ -787,10 +788,12 @@ struct ora_hit {
   float distance;
   float impact[3];
   float normal[3];
+  uint32_t boundary;  // 1: a boundary hit (see boundary_hit)
 };
 
 struct ora_counters {
-  uint64_t rays, stack_iterations, child_reads, brick_reads, voxel_tests, data_palette_reads, hits;
+  uint64_t rays, stack_iterations, child_reads, brick_reads, voxel_tests, data_palette_reads, hits,
+      boundary_hits;
 };
 
 void* ora_tree_load(const ora_dump* d) {
@@ -845,6 +848,26 @@ void ora_get_many(const void* tp, const uint32_t* xyz, uint64_t n, uint32_t* out
   for (uint64_t i = 0; i < n; ++i) out[i] = ora::get(t, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
 }
 
+// A boundary hit: the reference's f32 impact point lies within 1e-6 voxel
+// units of a second face of the hit cell (the voxel cell of a Parted brick,
+// the brick of a Solid one), so that the ray meets an edge or a corner of the
+// cell there and the cell it is taken to enter is decided by an exact tie of
+// f32 values. Per axis the distance to the nearer face of the cell is taken
+// in f64; the impact lies on (or next to) its entry face, the smallest of the
+// three; the second smallest must exceed 1e-6 for the hit cell to be
+// unambiguous. (For coordinates of 16 and more f32 values are 2^-19 apart or
+// more, so there the second face is hit exactly.)
+static uint32_t boundary_hit(const ora::Hit& h) {
+  const double eps = 1e-6;
+  const double p[3] = {h.impact.x, h.impact.y, h.impact.z};
+  const double m[3] = {h.cell.min_position.x, h.cell.min_position.y, h.cell.min_position.z};
+  const double u = h.cell.size;
+  double e[3];
+  for (int i = 0; i < 3; ++i) e[i] = std::min(std::fabs(p[i] - m[i]), std::fabs(m[i] + u - p[i]));
+  std::sort(e, e + 3);
+  return e[1] <= eps ? 1u : 0u;
+}
+
 static void cast_one(const ora::Octree& t, const float* r, ora_hit* out, ora::Counters* c,
                      float viewing_distance) {
   using namespace ora;
@@ -855,6 +878,8 @@ static void cast_one(const ora::Octree& t, const float* r, ora_hit* out, ora::Co
   const bool hit = get_by_ray(q, &h);
   out->hit = hit ? 1u : 0u;
   out->flags = q.flags;
+  out->boundary = hit ? boundary_hit(h) : 0u;
+  if (c && out->boundary) c->boundary_hits++;
   if (!hit) {
     out->pix = 0xFFFFFFFFu;
     out->albedo = 0;
@@ -903,6 +928,7 @@ void ora_cast_lod(const void* tp, const float* rays, uint64_t n, ora_hit* out, u
       counters->voxel_tests += c.voxel_tests;
       counters->data_palette_reads += c.data_palette_reads;
       counters->hits += c.hits;
+      counters->boundary_hits += c.boundary_hits;
     }
     return;
   }

@@ -38,6 +38,7 @@ def test_kat_fixed_rays(gpu_device):
     for name, (builder, r, expect) in fixed_kats().items():
         _say(f"KAT {name}")
         tree = builder()
+        data = tree.export_model()["data"]
         got = _cast(tree, r[None, :], gpu_device)
         exp = _oracle(tree).cast(r[None, :])
         assert_parity(exp, got, name)
@@ -51,6 +52,7 @@ def test_kat_fixed_rays(gpu_device):
             # voxel_data!(&5) is Informative: no colour index, data index set
             assert hit and got["albedo"][0] == 0, name
             assert (got["pix"][0] & 0xFFFF) == 0xFFFF and (got["pix"][0] >> 16) != 0xFFFF, name
+            assert int(data[int(got["pix"][0]) >> 16]) == expect[1], name
         elif kind == "albedo":
             assert hit and got["albedo"][0] == expect[1], name
         elif kind == "albedo_normal":
@@ -60,7 +62,8 @@ def test_kat_fixed_rays(gpu_device):
             assert hit and got["albedo"][0] == expect[1], name
             assert np.linalg.norm(got["normal"][0]) < expect[2], name
         elif kind == "none_or_data":
-            assert (not hit) or (got["pix"][0] & 0xFFFF) == 0xFFFF, name
+            assert (not hit) or ((got["pix"][0] & 0xFFFF) == 0xFFFF and
+                                 int(data[int(got["pix"][0]) >> 16]) == expect[1]), name
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -84,6 +87,8 @@ def test_kat_random_trees(gpu_device, seed):
         assert_parity(exp, got, f"{name}/{seed}")
         assert got["hit"].all(), f"{name}/{seed}: every ray at a filled voxel must hit"
         assert ((got["pix"] & 0xFFFF) == 0xFFFF).all()
+        data = tree.export_model()["data"]
+        assert (data[(got["pix"] >> 16).astype(np.int64)] == 5).all(), f"{name}/{seed}: data 5"
 
 
 def test_plain_render_all_rays(gpu_device):

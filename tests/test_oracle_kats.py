@@ -218,7 +218,8 @@ def test_dda_step_against_plane_intersection():
 def test_ray_kat_fixed(name):
     builder, r, expect = fixed_kats()[name]
     tree = builder()
-    h = oracle.OracleTree(tree.export_model()).cast(r[None, :])[0]
+    model = tree.export_model()
+    h = oracle.OracleTree(model).cast(r[None, :])[0]
     assert h["flags"] == 0
     kind = expect[0]
     if kind == "some":
@@ -226,7 +227,10 @@ def test_ray_kat_fixed(name):
     elif kind == "none":
         assert not h["hit"]
     elif kind == "data":
+        # voxel_data!(&5): no colour index, and the data palette entry the
+        # hit's data index names holds the value itself
         assert h["hit"] and h["albedo"] == 0 and (h["pix"] & 0xFFFF) == 0xFFFF
+        assert int(model["data"][int(h["pix"]) >> 16]) == expect[1]
     elif kind == "albedo":
         assert h["hit"] and h["albedo"] == expect[1]
     elif kind == "albedo_normal":
@@ -235,7 +239,8 @@ def test_ray_kat_fixed(name):
         assert h["hit"] and h["albedo"] == expect[1]
         assert np.linalg.norm(h["normal"]) < expect[2]
     elif kind == "none_or_data":
-        assert (not h["hit"]) or (h["pix"] & 0xFFFF) == 0xFFFF
+        assert (not h["hit"]) or ((h["pix"] & 0xFFFF) == 0xFFFF and
+                                  int(model["data"][int(h["pix"]) >> 16]) == expect[1])
 
 
 @pytest.mark.parametrize("seed", range(16))
@@ -252,10 +257,12 @@ def test_ray_kat_random(seed):
     for tree, filled, maker, off, check_get in cases:
         if not filled:
             continue
-        o = oracle.OracleTree(tree.export_model())
+        model = tree.export_model()
+        o = oracle.OracleTree(model)
         for p in filled:
             if check_get:
                 assert o.get(p) != 0xFFFFFFFF
             r = maker(np.array(p, np.float32) + np.float32(off), rng)
             h = o.cast(r[None, :])[0]
             assert h["hit"] and (h["pix"] & 0xFFFF) == 0xFFFF and h["flags"] == 0
+            assert int(model["data"][int(h["pix"]) >> 16]) == 5
