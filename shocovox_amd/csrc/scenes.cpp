@@ -15,6 +15,8 @@
 //   simplified to Empty / Solid as the reference's brick simplify does).
 // kind 3 (spheres): seeded spheres (radius 2-64) over a noise heightfield,
 //   built bottom-up; non-empty leaf bricks are Parted or Solid 50/50.
+// kind 4: the kind 1 heightfield built bottom-up (same voxels; usable at any
+//   size, where inserting voxel by voxel would take too long).
 // Bottom-up builds produce the reference's node model directly: Leaf nodes
 // with OccupancyBitmap occupancy, Internal nodes whose occupancy bits are the
 // union of their children's, and the reference's simplification rules
@@ -22,6 +24,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <climits>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -95,6 +98,74 @@ float value3(float x, float y, float z, float period, uint32_t octave, uint32_t 
   const float c11 = lerp(v(x0, y0 + 1, z0 + 1), v(x0 + 1, y0 + 1, z0 + 1), tx);
   return lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
 }
+
+// value3 over one brick with the lattice values it needs computed once:
+// fill() hashes the lattice points the brick [b, b + d)^3 reaches at one
+// octave, eval() is value3 at a point of the brick, the same operations in
+// the same order with the lattice values taken from the table.
+struct Value3Brick {
+  float period = 1.0f;
+  int32_t i0 = 0, j0 = 0, k0 = 0;
+  int32_t ni = 0, nj = 0, nk = 0;
+  std::vector<float> v;
+  void fill(float per, uint32_t octave, uint32_t seed, uint32_t bx, uint32_t by, uint32_t bz,
+            uint32_t d) {
+    period = per;
+    auto lo = [&](uint32_t b) { return static_cast<int32_t>(floorf(static_cast<float>(b) / per)); };
+    auto hi = [&](uint32_t b) {
+      return static_cast<int32_t>(floorf(static_cast<float>(b + d - 1) / per)) + 1;
+    };
+    i0 = lo(bx);
+    j0 = lo(by);
+    k0 = lo(bz);
+    ni = hi(bx) - i0 + 1;
+    nj = hi(by) - j0 + 1;
+    nk = hi(bz) - k0 + 1;
+    v.resize(static_cast<size_t>(ni) * nj * nk);
+    for (int32_t k = 0; k < nk; ++k)
+      for (int32_t j = 0; j < nj; ++j)
+        for (int32_t i = 0; i < ni; ++i)
+          v[i + ni * (j + nj * k)] =
+              unit_from(hash4(static_cast<uint32_t>(i0 + i) * 73856093u ^
+                                  static_cast<uint32_t>(j0 + j),
+                              static_cast<uint32_t>(k0 + k), octave + 16u, seed));
+  }
+  float eval(float x, float y, float z) const {
+    const float fx = x / period, fy = y / period, fz = z / period;
+    const float x0f = floorf(fx), y0f = floorf(fy), z0f = floorf(fz);
+    const int32_t x0 = static_cast<int32_t>(x0f) - i0, y0 = static_cast<int32_t>(y0f) - j0,
+                  z0 = static_cast<int32_t>(z0f) - k0;
+    const float tx = smooth(fx - x0f), ty = smooth(fy - y0f), tz = smooth(fz - z0f);
+    auto at = [&](int32_t i, int32_t j, int32_t k) { return v[i + ni * (j + nj * k)]; };
+    auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+    const float c00 = lerp(at(x0, y0, z0), at(x0 + 1, y0, z0), tx);
+    const float c10 = lerp(at(x0, y0 + 1, z0), at(x0 + 1, y0 + 1, z0), tx);
+    const float c01 = lerp(at(x0, y0, z0 + 1), at(x0 + 1, y0, z0 + 1), tx);
+    const float c11 = lerp(at(x0, y0 + 1, z0 + 1), at(x0 + 1, y0 + 1, z0 + 1), tx);
+    return lerp(lerp(c00, c10, ty), lerp(c01, c11, ty), tz);
+  }
+};
+
+// fbm3 over one brick (the four octaves' lattice tables)
+struct Fbm3Brick {
+  Value3Brick oct[4];
+  void fill(float period, uint32_t seed, uint32_t bx, uint32_t by, uint32_t bz, uint32_t d) {
+    for (uint32_t o = 0; o < 4; ++o) {
+      oct[o].fill(period, o, seed, bx, by, bz, d);
+      period *= 0.5f;
+    }
+  }
+  // fbm3's sum, in fbm3's order
+  float eval(float x, float y, float z) const {
+    float sum = 0.0f, amp = 1.0f, norm = 0.0f;
+    for (uint32_t o = 0; o < 4; ++o) {
+      sum += amp * oct[o].eval(x, y, z);
+      norm += amp;
+      amp *= 0.5f;
+    }
+    return sum / norm;
+  }
+};
 
 float fbm3(float x, float y, float z, float period, uint32_t seed) {
   float sum = 0.0f, amp = 1.0f, norm = 0.0f;
@@ -413,13 +484,19 @@ Status build_caves(Octree& t, uint64_t seed) {
     plan.kind = BrickKind::Parted;
     plan.voxels.assign(static_cast<size_t>(d) * d * d, EMPTY_MARKER);
     const auto& near = grid.at(lx, ly, lz);
+    // fbm3 and value3 of the brick's voxels with each lattice point hashed
+    // once (the same values as evaluating them point by point)
+    thread_local Fbm3Brick density;
+    thread_local Value3Brick tint_noise;
+    density.fill(48.0f, sd, bx, by, bz, d);
+    tint_noise.fill(8.0f, 5u, sd, bx, by, bz, d);
     for (uint32_t z = 0; z < d; ++z)
       for (uint32_t y = 0; y < d; ++y)
         for (uint32_t x = 0; x < d; ++x) {
           const float px = static_cast<float>(bx + x), py = static_cast<float>(by + y),
                       pz = static_cast<float>(bz + z);
           // porous density: denser at the bottom, sparser towards the top
-          const float dens = fbm3(px, py, pz, 48.0f, sd);
+          const float dens = density.eval(px, py, pz);
           if (dens < 0.40f + 0.22f * (py / fs)) continue;
           bool carved = false;
           for (uint32_t si : near) {
@@ -431,8 +508,7 @@ Status build_caves(Octree& t, uint64_t seed) {
             }
           }
           if (carved) continue;
-          const uint32_t tint =
-              static_cast<uint32_t>(4.0f * value3(px, py, pz, 8.0f, 5u, sd)) & 3u;
+          const uint32_t tint = static_cast<uint32_t>(4.0f * tint_noise.eval(px, py, pz)) & 3u;
           plan.voxels[x + y * d + z * d * d] = pix[(band(by + y) + tint) & 15u];
         }
   });
@@ -517,6 +593,65 @@ Status build_spheres(Octree& t, uint64_t seed) {
   return Status::Ok;
 }
 
+// The heightfield content of kind 1 built bottom-up (kind 4): the same
+// heights and band colours, one height evaluation per column, for trees too
+// large to build voxel by voxel with insert.
+Status build_heightfield_bottom_up(Octree& t, uint64_t seed) {
+  const uint32_t s = t.size(), d = t.brick_dim();
+  const float period = static_cast<float>(s) / 4.0f;
+  const uint32_t sd = static_cast<uint32_t>(seed ^ (seed >> 32));
+  std::vector<int32_t> height(static_cast<size_t>(s) * s);
+  parallel_for(s, [&](size_t x) {
+    for (uint32_t z = 0; z < s; ++z) {
+      const float h = static_cast<float>(s) / 8.0f +
+                      (3.0f * static_cast<float>(s) / 4.0f) *
+                          fbm2(static_cast<float>(x), static_cast<float>(z), period, sd);
+      height[x + static_cast<size_t>(z) * s] = static_cast<int32_t>(floorf(h));
+    }
+  });
+  std::vector<uint32_t> pix(16);
+  for (uint32_t i = 0; i < 16; ++i)
+    pix[i] = t.add_to_palette(Entry{EntryKind::Visual, PALETTE16[i], 0});
+  auto band = [&](uint32_t y) {
+    const uint32_t b = static_cast<uint32_t>((static_cast<uint64_t>(y) * 16u) / s);
+    return b > 15u ? 15u : b;
+  };
+  BottomUpBuilder b(t);
+  b.build([&](uint32_t lx, uint32_t ly, uint32_t lz, uint32_t o, LeafBrickPlan& plan) {
+    const uint32_t bx = lx * 2 * d + (o & 1u) * d, by = ly * 2 * d + ((o >> 2) & 1u) * d,
+                   bz = lz * 2 * d + ((o >> 1) & 1u) * d;
+    int32_t hmin = INT32_MAX, hmax = INT32_MIN;
+    for (uint32_t z = 0; z < d; ++z)
+      for (uint32_t x = 0; x < d; ++x) {
+        const int32_t h = height[(bx + x) + static_cast<size_t>(bz + z) * s];
+        hmin = std::min(hmin, h);
+        hmax = std::max(hmax, h);
+      }
+    if (static_cast<int32_t>(by) > hmax) {
+      plan.kind = BrickKind::Empty;
+      return;
+    }
+    if (static_cast<int32_t>(by + d - 1) <= hmin && band(by) == band(by + d - 1)) {
+      // below the surface in one colour band: the Solid brick the builder's
+      // simplification would make of the filled Parted one
+      plan.kind = BrickKind::Solid;
+      plan.solid = pix[band(by)];
+      return;
+    }
+    plan.kind = BrickKind::Parted;
+    plan.voxels.assign(static_cast<size_t>(d) * d * d, EMPTY_MARKER);
+    for (uint32_t z = 0; z < d; ++z)
+      for (uint32_t y = 0; y < d; ++y)
+        for (uint32_t x = 0; x < d; ++x) {
+          const int32_t h = height[(bx + x) + static_cast<size_t>(bz + z) * s];
+          if (static_cast<int32_t>(by + y) <= h)
+            plan.voxels[x + y * d + z * d * d] = pix[band(by + y)];
+        }
+    // (bricks that come out homogeneous are simplified by the builder)
+  });
+  return Status::Ok;
+}
+
 }  // namespace
 
 Status build_scene(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t seed, Octree* out) {
@@ -530,6 +665,8 @@ Status build_scene(uint32_t kind, uint32_t size, uint32_t brick_dim, uint64_t se
       return build_caves(*out, seed);
     case 3:
       return build_spheres(*out, seed);
+    case 4:
+      return build_heightfield_bottom_up(*out, seed);
     default:
       return Status::InvalidStructure;
   }
