@@ -25,16 +25,24 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = ("Mrays/s at 1080p into 512³; p50/p99 latency batched vs per-call at fixed QPS")
 HBM_PEAK_GBS = 8000.0
+# VALU issue ceiling of an MI355X in G wave64 instructions per second: each
+# of a CU's 4 SIMDs is 16 lanes wide and issues one wave64 VALU instruction
+# every 4 cycles, so a CU completes one per cycle; 256 CUs at the 2.4 GHz
+# peak engine clock.
+VALU_PEAK_GIPS = 256 * 1.0 * 2.4
 # Algorithmic bytes of one logical event of the reference traversal
 EVENT_BYTES = {"stack_iterations": 12, "child_reads": 4, "brick_reads": 4, "voxel_tests": 8,
                "data_palette_reads": 4, "hits": 4}
@@ -45,10 +53,44 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, tree, vps, seconds: float, extra_rays=None) -> tuple[dict, dict]:
+def step_parity(otree, rays, outs, extra=None) -> dict:
+    """Every ray of the measured step against the oracle: `outs` are the
+    step's device outputs (hit, albedo, voxel, distance), one dict per render
+    job in ray order, and `extra` = (rays, outputs) of a cast job. Reports the
+    rays whose fields differ and the boundary hits (impact within 1e-6 of a
+    second face of the hit cell, where the hit cell is decided by an exact
+    f32 tie), and how many of those differ."""
+    import numpy as np
+
+    cores = min(16, os.cpu_count() or 1)
+    groups = [(rays, outs)] + ([extra] if extra is not None else [])
+    checked = bad = boundary = bad_boundary = 0
+    for rr, oo in groups:
+        exp = otree.cast(rr, threads=cores)
+        got = {k: np.concatenate([o[k].cpu().numpy().reshape(-1) for o in oo]) for k in
+               ("hit", "albedo", "voxel", "distance")}
+        n = len(exp)
+        differ = np.zeros(n, bool)
+        differ |= got["hit"].view(np.uint32) != exp["hit"]
+        differ |= got["albedo"].view(np.uint32) != exp["albedo"]
+        differ |= (got["voxel"].view(np.uint32).reshape(n, 3) != exp["voxel"]).any(axis=1)
+        differ |= got["distance"].view(np.uint32) != exp["distance"].view(np.uint32)
+        b = exp["boundary"] == 1
+        checked += n
+        bad += int(differ.sum())
+        boundary += int(b.sum())
+        bad_boundary += int((differ & b).sum())
+    return {"rays_checked": checked, "fields": "hit, albedo, voxel xyz, distance (bits)",
+            "mismatching_rays": bad, "boundary_hits": boundary,
+            "boundary_fraction": boundary / max(1, checked),
+            "boundary_mismatches": bad_boundary}
+
+
+def cpu_baseline(cfg, tree, vps, seconds: float, extra_rays=None):
     """Times the CPU restatement (single thread, then all cores) on a bounded
     sample of the same rays and returns (cpu_baseline, per-ray event counts of
-    the step's rays: the primary rays plus `extra_rays` if given)."""
+    the step's rays: the primary rays plus `extra_rays` if given, the loaded
+    oracle tree, the primary rays)."""
     import numpy as np
 
     import oracle
@@ -96,7 +138,7 @@ def cpu_baseline(cfg, tree, vps, seconds: float, extra_rays=None) -> tuple[dict,
         "all_cores": {"value": round(len(sample) / tn / 1e6, 3), "cores": cores},
         "cpu_model": cpu_model,
     }
-    return base, per_ray
+    return base, per_ray, otree, rays
 
 
 def copy_calibration(args) -> int:
@@ -121,43 +163,32 @@ def copy_calibration(args) -> int:
     return 0
 
 
-BATCH_RESOLUTIONS = ((128, 128), (512, 512), (1280, 720), (1920, 1080))
-
-
 def batched_bench(args, render_variant: str) -> dict:
-    """Batched mode: trees of uneven size (log-uniform 64..max), each with 1-4
-    viewports of resolution drawn from {128^2, 512^2, 720p, 1080p}; all rays in
-    one persistent launch per batch kernel variant (a replayed hipGraph),
-    against one plain render launch per viewport. Outputs of the two forms are
-    compared for equality."""
-    import math
-
-    import numpy as np
+    """Batched mode: trees of uneven size (log-uniform 64..max), content of the
+    plain heightfield (built bottom-up) or the caves by seed, brick dims
+    4/8/16, each with 1-4 viewports of resolution drawn from {128^2, 512^2,
+    720p, 1080p}; all rays in one persistent launch per batch kernel variant
+    (a replayed hipGraph), against one plain render launch per viewport.
+    Outputs of the two forms are compared for equality."""
     import torch
 
     from shocovox_amd import (Batch, DeviceTree, Octree, alloc_hits, batch_variants,
                               select_batch_variant, workloads as W)
 
-    rng = np.random.default_rng(args.seed + 700)
-    max_exp = int(np.log2(args.batched_max_size))
     t0 = time.time()
-    trees, dtrees, views = [], [], []
-    for i in range(args.batched_trees):
-        size = int(2 ** rng.integers(6, max_exp + 1))
-        dim = int(rng.choice([4, 8, 16]))
-        kind = int(rng.integers(1, 3)) if size <= 128 else 2
-        t = Octree.scene(kind, size, dim, args.seed * 1000 + 500 + i)
+    specs, views = W.batched_pool_specs(args.seed, args.batched_trees, args.batched_max_size)
+    trees, dtrees = [], []
+    kinds = {2: 0, 4: 0}
+    for kind, size, dim, seed in specs:
+        kinds[kind] += 1
+        t = Octree.scene(kind, size, dim, seed)
         trees.append(t)
         dtrees.append(DeviceTree(t))
-        c = size / 2.0
-        for _ in range(int(rng.integers(1, 5))):
-            w, h = BATCH_RESOLUTIONS[int(rng.integers(0, len(BATCH_RESOLUTIONS)))]
-            az = float(rng.uniform(0, 2 * math.pi))
-            eye = (c + 1.3 * size * math.cos(az), 0.9 * size, c + 1.3 * size * math.sin(az))
-            views.append((i, W.look_at_viewport(eye, (c, c / 2, c), w, h), w, h))
     rays = sum(w * h for _, _, w, h in views)
-    log(f"batched: {len(trees)} trees, {len(views)} viewports, {rays} rays, built in "
-        f"{time.time() - t0:.1f}s")
+    pool_bytes = sum(d.device_bytes for d in dtrees)
+    sizes = [t.get_size() for t in trees]
+    log(f"batched: {len(trees)} trees ({pool_bytes / 2**30:.2f} GiB), {len(views)} viewports, "
+        f"{rays} rays, built in {time.time() - t0:.1f}s")
     dev = torch.device("cuda:0")
     fields = ("hit", "albedo", "voxel", "distance")
     plain_bufs = [alloc_hits(w * h, dev, fields) for _, _, w, h in views]
@@ -176,15 +207,20 @@ def batched_bench(args, render_variant: str) -> dict:
         plain_pass()
     torch.cuda.synchronize()
     plain_s = (time.perf_counter() - t1) / reps
-    out = {"workload": (f"{len(trees)} trees (sizes 64..{args.batched_max_size}, brick dims "
-                        f"4/8/16), {len(views)} viewports of 128^2/512^2/720p/1080p"),
-           "trees": len(trees), "viewports": len(views), "rays": rays,
+    out = {"workload": (f"{len(trees)} trees (sizes {min(sizes)}..{max(sizes)} log-uniform, "
+                        f"{sizes.count(1024)} of 1024^3; brick dims 4/8/16; {kinds[2]} caves, "
+                        f"{kinds[4]} heightfield; {pool_bytes / 2**30:.2f} GiB resident), "
+                        f"{len(views)} viewports of 128^2/512^2/720p/1080p, {rays} rays"),
+           "trees": len(trees), "tree_sizes": sorted(sizes), "pool_bytes": pool_bytes,
+           "viewports": len(views), "rays": rays,
            "plain": {"ms": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 1),
                      "launches": len(views), "render_variant": render_variant},
            "batched": {}}
     names = batch_variants()
     best = None
     for vi, name in enumerate(names):
+        if name.startswith("r1-") and not args.batched_all_variants:
+            continue  # lane refill: measured 2-3x slower; timed with --batched-all-variants
         select_batch_variant(vi)
         batch = Batch(len(views))
         jobs = [Batch.render_job(dtrees[ti], vp, w, h, b)
@@ -210,106 +246,194 @@ def batched_bench(args, render_variant: str) -> dict:
     out["best_variant"] = best
     if best:
         out["speedup_vs_plain"] = round(plain_s / (out["batched"][best]["ms"] / 1e3), 3)
+    del dtrees, trees
+    torch.cuda.synchronize()
     return out
 
 
+def served_pool(args):
+    """(specs, trees): the served pool's trees, built on the host. Sizes are
+    log-uniform over 64..max (stratified), brick dims 4/8/16 below 1024^3 and
+    16 from 1024^3 on; content by seed: caves or the plain heightfield below
+    2048^3, spheres over a heightfield or the heightfield at 2048^3, any of
+    the three at 1024^3."""
+    from shocovox_amd import Octree, workloads as W
+
+    specs = W.served_pool_specs(args.seed, args.served_trees, args.served_max_size)
+    t0 = time.time()
+    trees = []
+    for k, (kind, size, dim, seed) in enumerate(specs):
+        trees.append(Octree.scene(kind, size, dim, seed))
+        if size >= 1024:
+            log(f"served pool: tree {k} ({size}^3, kind {kind}) built at {time.time() - t0:.1f}s")
+    return specs, trees
+
+
+def admit_pool(trees, budget: int):
+    """Admits the trees in order while their device bytes fit the pool budget
+    (itself at most the free device memory less the loop's reserve); returns
+    the device trees (None where refused) and the bytes admitted."""
+    from shocovox_amd import serve
+
+    dtrees, total = [], 0
+    for t in trees:
+        if budget - total <= 0:
+            log(f"served: tree {t.get_size()}^3 refused: the pool budget is spent")
+            dtrees.append(None)
+            continue
+        try:
+            d = serve.admit(t, budget_bytes=budget - total, reserve_bytes=0)
+        except Exception as e:  # admission refused: the tree stays on the host
+            log(f"served: tree {t.get_size()}^3 refused: {e}")
+            dtrees.append(None)
+            continue
+        dtrees.append(d)
+        total += d.device_bytes
+    return dtrees, total
+
+
 def served_bench(args, capacity_mrays: float) -> dict:
-    """Served mode: a pool of resident trees admitted by free HBM, then the
-    same seeded Poisson plan (Zipf(1.1) over trees and resolutions, 5% sparse
-    edits of 1-1000 voxels) served by continuous batching (mode "batched",
-    which runs a launch holding one whole request as a plain render, and
-    "batched-kernel", which never does) and by per-request plain launches, at
-    fixed fractions of the measured render capacity."""
-    import numpy as np
+    """Served mode over a pool of resident trees admitted by free HBM (and a
+    budget): the same seeded Poisson plans (Zipf(1.1) over trees and
+    resolutions, 5% sparse edits of 1-1000 voxels) served by per-request
+    plain launches and by continuous batching, each mode on its own copy of
+    the pool (sessions edit their trees). Per mode: p50/p99 at fixed
+    fractions of a nominal capacity, then the largest offered load whose p99
+    stays within a latency target (the plain mode's p99 at the target load),
+    found by search; the throughput ratio of the modes is taken at that
+    equal p99."""
+    import torch
 
-    from shocovox_amd import Octree, batch_variants, current_batch_variant, serve
+    from shocovox_amd import batch_variants, current_batch_variant, serve
 
-    rng = np.random.default_rng(args.seed + 500)
-    max_exp = int(np.log2(args.served_max_size))
-    specs = []
-    for i in range(args.served_trees):
-        size = int(2 ** rng.integers(6, max_exp + 1))  # log-uniform over 64..max
-        dim = int(rng.choice([4, 8, 16]))
-        kind = int(rng.integers(1, 3)) if size <= 128 else 2
-        specs.append((kind, size, dim, int(args.seed * 1000 + i)))
+    t0 = time.time()
+    specs, pristine = served_pool(args)
+    log(f"served: pool of {len(pristine)} trees built in {time.time() - t0:.1f}s")
+    # device memory kept free for the loop's output pools and a margin
     cfg0 = serve.ServeConfig(qps=1.0, duration_s=1.0, seed=args.seed,
                              max_inflight=args.served_inflight,
                              max_batch_rays=args.served_batch_rays)
-    modes = ("batched", "plain", "batched-kernel")
-    # device memory kept free for the loop's output pools and a margin; every
-    # mode gets its own copy of the pool, so each starts from the same trees
-    # and sees the same edits in the same order (sessions mutate the trees)
-    reserve = (cfg0.max_inflight * max(cfg0.max_batch_rays, 3840 * 2160) * 24) + (1 << 30)
-    t0 = time.time()
-    trees, dtrees, refused, pool_bytes = [], [], 0, 0
-    for kind, size, dim, seed in specs:
-        t = Octree.scene(kind, size, dim, seed)
-        try:
-            d = serve.admit(t, reserve_bytes=reserve)
-        except Exception as e:  # admission refused: the tree stays on the host
-            log(f"served: tree {size}^3 refused: {e}")
-            refused += 1
-            continue
-        trees.append(t)
-        dtrees.append(d)
-        pool_bytes += d.device_bytes
-    log(f"served: {len(trees)} trees resident ({pool_bytes / 2**30:.2f} GiB) in "
-        f"{time.time() - t0:.1f}s, {refused} refused")
-    pools = {modes[0]: (trees, dtrees)}
-    for mode in modes[1:]:
-        copies = [t.clone() for t in trees]
-        pools[mode] = (copies, [serve.admit(t, reserve_bytes=reserve) for t in copies])
-    sizes = [t.get_size() for t in trees]
-    probe = serve.plan(serve.ServeConfig(qps=1000.0, duration_s=20.0, seed=args.seed), sizes)
-    mean_rays = sum(r.width * r.height for r in probe if r.kind == 0) / len(probe)
-    qps_cap = capacity_mrays * 1e6 / mean_rays
-    # Untimed warm-up session per pool (the served counterpart of --warmup):
-    # the first edits of a tree grow its device arrays and fault in its host
-    # mirror, a one-time cost that would otherwise land on one measured session.
-    t0 = time.time()
-    for mode, (ptrees, pdtrees) in pools.items():
-        serve.run(ptrees, pdtrees, serve.ServeConfig(
-            qps=round(0.6 * qps_cap, 1), duration_s=min(1.0, args.served_seconds),
-            seed=args.seed + 7, mode=mode, max_inflight=args.served_inflight,
-            max_batch_rays=args.served_batch_rays))
-    log(f"served: warm-up sessions in {time.time() - t0:.1f}s")
-    levels = []
-    for frac in args.served_loads:
-        qps = round(frac * qps_cap, 1)
-        row = {"load": frac, "qps": qps}
-        for mode in modes:
-            ptrees, pdtrees = pools[mode]
-            cfg = serve.ServeConfig(qps=qps, duration_s=args.served_seconds, seed=args.seed,
+    reserve = (cfg0.max_inflight * max(cfg0.max_batch_rays, 3840 * 2160) * 24) + (2 << 30)
+    free, total = serve.device_memory()
+    budget = min(int(args.served_budget_gb * 2 ** 30), free - reserve)
+    modes = ("plain", "batched")
+    if args.served_kernel_mode:
+        modes += ("batched-kernel",)
+    results = {}
+    admitted = refused = pool_bytes = None
+    levels = {}
+    for mi, mode in enumerate(modes):
+        # every mode but the last runs on a copy of the pristine pool
+        t1 = time.time()
+        ptrees = [t.clone() for t in pristine] if mi < len(modes) - 1 else pristine
+        dtrees, nbytes = admit_pool(ptrees, budget)
+        keep = [i for i, d in enumerate(dtrees) if d is not None]
+        if admitted is None:
+            admitted, refused, pool_bytes = keep, len(ptrees) - len(keep), nbytes
+        elif keep != admitted:
+            raise SystemExit("served: admission differs between modes")
+        ptrees = [ptrees[i] for i in keep]
+        dtrees = [dtrees[i] for i in keep]
+        log(f"served {mode}: {len(dtrees)} trees admitted ({nbytes / 2**30:.2f} GiB, "
+            f"{len(specs) - len(keep)} refused) in {time.time() - t1:.1f}s")
+        sizes = [t.get_size() for t in ptrees]
+        probe = serve.plan(serve.ServeConfig(qps=1000.0, duration_s=20.0, seed=args.seed), sizes)
+        mean_rays = sum(r.width * r.height for r in probe if r.kind == 0) / len(probe)
+        qps_cap = capacity_mrays * 1e6 / mean_rays
+
+        def session(load, seconds, seed=args.seed):
+            cfg = serve.ServeConfig(qps=round(load * qps_cap, 1), duration_s=seconds, seed=seed,
                                     mode=mode, max_inflight=args.served_inflight,
                                     max_batch_rays=args.served_batch_rays)
-            # a mode's sessions run back to back on its pool; their edits accumulate
-            r = serve.run(ptrees, pdtrees, cfg)
-            row[mode] = {"mrays_per_s": round(r["mrays_per_s"], 1),
-                         "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
-                         "render_p99_ms": round(r["render_p99_ms"], 3),
-                         "edit_p99_ms": round(r["edit_p99_ms"], 3),
-                         "edit_phase_p99_ms": {k: round(r[f"edit_{k}_p99_ms"], 3)
-                                               for k in ("wait", "apply", "upload")},
-                         "requests": int(r["requests"]), "edits": int(r["edits"]),
-                         "launches": int(r["launches"]),
-                         "jobs_per_launch": round(r["mean_jobs_per_launch"], 2),
-                         "single_launches": int(r["single_launches"]),
-                         "elapsed_s": round(r["elapsed_s"], 3)}
-            log(f"served load {frac} ({qps} req/s) {mode}: {row[mode]}")
-        b, p = row["batched"], row["plain"]
-        row["throughput_ratio"] = round(b["mrays_per_s"] / max(p["mrays_per_s"], 1e-9), 3)
-        levels.append(row)
+            r = serve.run(ptrees, dtrees, cfg)
+            return {"load": load, "qps": cfg.qps, "mrays_per_s": round(r["mrays_per_s"], 1),
+                    "p50_ms": round(r["p50_ms"], 3), "p99_ms": round(r["p99_ms"], 3),
+                    "render_p99_ms": round(r["render_p99_ms"], 3),
+                    "edit_p99_ms": round(r["edit_p99_ms"], 3),
+                    "edit_phase_p99_ms": {k: round(r[f"edit_{k}_p99_ms"], 3)
+                                          for k in ("wait", "apply", "upload")},
+                    "requests": int(r["requests"]), "edits": int(r["edits"]),
+                    "launches": int(r["launches"]),
+                    "jobs_per_launch": round(r["mean_jobs_per_launch"], 2),
+                    "single_launches": int(r["single_launches"]),
+                    "elapsed_s": round(r["elapsed_s"], 3)}
+
+        # untimed warm-up: the first edits of a tree grow its device arrays
+        session(0.6, min(1.0, args.served_seconds), seed=args.seed + 7)
+        rows = []
+        for frac in args.served_loads:
+            row = session(frac, args.served_seconds)
+            log(f"served {mode} load {frac}: {row}")
+            rows.append(row)
+            levels.setdefault(frac, {})[mode] = row
+        if mode == "plain":
+            ref = min(args.served_loads, key=lambda f: abs(f - args.served_slo_load))
+            slo = levels[ref]["plain"]["p99_ms"]
+        # the largest load whose p99 stays within the target: grow by 1.3x
+        # until it does not, then two bisection steps
+        lo, hi, probes = None, None, []
+        load = 0.5
+        while load <= args.served_max_load:
+            row = session(load, args.served_seconds)
+            probes.append(row)
+            log(f"served {mode} search load {load:.3f}: p99 {row['p99_ms']} ms (target {slo})")
+            if row["p99_ms"] <= slo:
+                lo = row
+                load *= 1.3
+            else:
+                hi = row
+                break
+        for _ in range(2 if lo is not None and hi is not None else 0):
+            load = math.sqrt(lo["load"] * hi["load"])
+            row = session(load, args.served_seconds)
+            probes.append(row)
+            log(f"served {mode} search load {load:.3f}: p99 {row['p99_ms']} ms (target {slo})")
+            if row["p99_ms"] <= slo:
+                lo = row
+            else:
+                hi = row
+        results[mode] = {"rows": rows, "search": probes, "at_target": lo,
+                         "capacity_qps": lo["qps"] if lo else None,
+                         "capacity_mrays_per_s": lo["mrays_per_s"] if lo else None,
+                         "capped": lo is not None and hi is None}
+        del dtrees
+        torch.cuda.synchronize()
+        if mi < len(modes) - 1:
+            del ptrees
+    p, b = results["plain"], results["batched"]
+    ratio = (round(b["capacity_mrays_per_s"] / p["capacity_mrays_per_s"], 3)
+             if p["capacity_mrays_per_s"] and b["capacity_mrays_per_s"] else None)
+    fixed = []
+    for frac in args.served_loads:
+        row = {"load": frac, **{m: levels[frac][m] for m in modes}}
+        row["throughput_ratio"] = round(levels[frac]["batched"]["mrays_per_s"] /
+                                        max(levels[frac]["plain"]["mrays_per_s"], 1e-9), 3)
+        fixed.append(row)
+    sizes = [specs[i][1] for i in admitted]
     return {
-        "workload": (f"{len(trees)} resident trees (sizes {min(sizes)}..{max(sizes)}, brick dims "
-                     "4/8/16), Poisson arrivals, Zipf(1.1) over tree and resolution "
-                     "{256^2, 720p, 1080p, 4K}, 5% edits of 1..1000 voxels"),
-        "trees": len(trees), "refused": refused, "pool_bytes": pool_bytes,
-        "mean_rays_per_request": round(mean_rays), "capacity_qps": round(qps_cap, 1),
+        "workload": (f"{len(admitted)} resident trees of {len(specs)} (sizes "
+                     f"{min(sizes)}..{max(sizes)} log-uniform, {sizes.count(2048)} of 2048^3; "
+                     f"{pool_bytes / 2**30:.2f} GiB admitted by free HBM within a "
+                     f"{args.served_budget_gb:g} GiB budget, {refused} refused), Poisson "
+                     "arrivals, Zipf(1.1) over tree and resolution {256^2, 720p, 1080p, 4K}, "
+                     "5% edits of 1..1000 voxels"),
+        "trees": len(admitted), "refused": refused, "pool_bytes": pool_bytes,
+        "tree_specs": [list(specs[i]) for i in admitted],
+        "device_total_bytes": total,
         "max_inflight": args.served_inflight, "max_batch_rays": args.served_batch_rays,
         "batch_variant": batch_variants()[current_batch_variant()],
         "arrival_seconds": args.served_seconds,
+        "nominal_capacity_qps": round(qps_cap, 1),
+        "p99_target_ms": slo, "p99_target_from": f"plain mode's p99 at load {ref}",
+        "capacity_at_target": {m: {"qps": results[m]["capacity_qps"],
+                                   "mrays_per_s": results[m]["capacity_mrays_per_s"],
+                                   "p99_ms": results[m]["at_target"]["p99_ms"]
+                                   if results[m]["at_target"] else None,
+                                   "search_capped": results[m]["capped"]} for m in modes},
+        "throughput_ratio_at_equal_p99": ratio,
+        "levels": fixed,
+        "search": {m: results[m]["search"] for m in modes},
         "warmup_session": "1 s at load 0.6 per mode, each mode on its own copy of the pool",
-        "levels": levels,
     }
 
 
@@ -334,18 +458,28 @@ def main() -> int:
                          "the fastest before the timed region")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-mode comparison")
     ap.add_argument("--batched-trees", type=int, default=64)
-    ap.add_argument("--batched-max-size", type=int, default=512)
+    ap.add_argument("--batched-max-size", type=int, default=1024)
     ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--batched-all-variants", action="store_true",
+                    help="time the lane-refill batch variants too")
     ap.add_argument("--no-served", action="store_true", help="skip the served-mode sessions")
     ap.add_argument("--served-trees", type=int, default=32)
-    ap.add_argument("--served-max-size", type=int, default=512)
-    ap.add_argument("--served-seconds", type=float, default=2.0)
+    ap.add_argument("--served-max-size", type=int, default=2048)
+    ap.add_argument("--served-budget-gb", type=float, default=40.0,
+                    help="device memory the served pool may take (admission stops there)")
+    ap.add_argument("--served-seconds", type=float, default=1.5)
     ap.add_argument("--served-inflight", type=int, default=2)
     # launches of 8M rays: a request of up to 4K completes in one launch, so a
     # popular tree goes idle often enough for its edits to start promptly
     ap.add_argument("--served-batch-rays", type=int, default=1 << 23)
     ap.add_argument("--served-loads", type=float, nargs="+", default=[0.3, 0.6, 0.9, 1.2],
                     help="offered load as fractions of the measured render capacity")
+    ap.add_argument("--served-slo-load", type=float, default=0.6,
+                    help="the p99 target is the plain mode's p99 at this load")
+    ap.add_argument("--served-max-load", type=float, default=4.0,
+                    help="upper end of the capacity search, as a fraction of capacity")
+    ap.add_argument("--served-kernel-mode", action="store_true",
+                    help="also serve with the persistent kernel for every launch")
     ap.add_argument("--mode", default="render", choices=["render", "copy"],
                     help="'copy' runs a plain device-to-device copy of known size (for "
                          "calibrating rocprofv3 byte counters)")
@@ -501,8 +635,9 @@ def main() -> int:
 
     roofline = None
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
-        cpu, per_ray = cpu_baseline(cfg, tree, vps, args.cpu_seconds, inc_host)
+        cpu, per_ray, otree, prays = cpu_baseline(cfg, tree, vps, args.cpu_seconds, inc_host)
         b_ray = sum(EVENT_BYTES[k] * v for k, v in per_ray.items()) + OUT_BYTES_PER_RAY
         # the timed unit: one batched launch (all rays of the step), or one
         # per-frame render launch (the render rays of one pose)
@@ -515,8 +650,11 @@ def main() -> int:
         achieved = b_ray * rays_per_launch / (launch_ms * 1e-3) / 1e9
         traffic = None
         counters = {}
+        issue = None
+        measured_frac = None
         # HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
-        # workload, scaled by the same counters' reading of a copy of known size
+        # workload, scaled by the same counters' reading of a copy of known size;
+        # VALU instructions per launch from a SQ_INSTS_VALU run
         tf = ROOT / "profiles" / f"{kname}_{cfg.name}_{cfg.height}p_traffic.json"
         if tf.exists():
             tj = json.loads(tf.read_text())
@@ -524,14 +662,39 @@ def main() -> int:
                 traffic = tj.get("bytes_per_launch")
                 counters = {k: tj[k] for k in ("l2_hit_rate", "valu_lane_utilization")
                             if k in tj}
+                if traffic:
+                    measured_frac = traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                if tj.get("valu_insts_per_launch"):
+                    gips = tj["valu_insts_per_launch"] / (launch_ms * 1e-3) / 1e9
+                    issue = {"achieved": round(gips, 1), "peak": VALU_PEAK_GIPS,
+                             "unit": "G wave64 VALU instructions/s",
+                             "frac": round(gips / VALU_PEAK_GIPS, 4),
+                             "valu_insts_per_launch": tj["valu_insts_per_launch"],
+                             "counters_from": tf.name}
+        # the limiter named by the counters: whichever ceiling the kernel is
+        # closest to, VALU issue or measured HBM bytes
+        limiter = None
+        if issue is not None or measured_frac is not None:
+            limiter = ("valu-issue" if (issue["frac"] if issue else 0.0) >= (measured_frac or 0.0)
+                       else "hbm")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "kernel": kname,
+                    "frac_kind": "logical: algorithmic bytes of the reference's event "
+                                 "sequence per ray x rays / launch time",
+                    "measured_hbm_frac": (round(measured_frac, 4) if measured_frac is not None
+                                          else None),
+                    "limiter": limiter, "issue": issue,
                     "bytes_per_ray": round(b_ray, 1),
                     "events_per_ray": {k: round(v, 3) for k, v in per_ray.items()},
                     "avg_launch_ms": round(launch_ms, 4),
                     "rays_per_launch": rays_per_launch, **counters}
         log(f"cpu baseline {cpu}")
+        t0 = time.time()
+        parity = step_parity(otree, prays, bufs,
+                             (inc_host, [incoherent[1]]) if incoherent is not None else None)
+        log(f"step parity against the oracle ({time.time() - t0:.1f}s): {parity}")
+        del otree
 
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
@@ -547,9 +710,11 @@ def main() -> int:
                    "device_bytes": dtree.device_bytes, "launch": launch_desc,
                    "batch_variant": bvariant, "batch_autotune_ms": batch_tuning or None,
                    "outputs_equal_to_per_frame": equal, "per_frame": per_frame,
-                   "launch_variant": best, "autotune_ms": tuning or None},
+                   "launch_variant": best, "autotune_ms": tuning or None,
+                   "tree": W.tree_statistics(tree)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     if not args.no_batched:
         line["batched"] = batched_bench(args, best)

@@ -236,8 +236,63 @@ int hip_fail(hipError_t e, const char* what, std::string* err) {
   return e == hipErrorOutOfMemory ? SVO_OUT_OF_DEVICE_MEMORY : SVO_HIP_ERROR;
 }
 
-size_t trees_offset_for(uint32_t njobs) {
+constexpr size_t trees_offset_for(uint32_t njobs) {
   return (kJobsOffset + njobs * sizeof(BatchJob) + 63) & ~size_t(63);
+}
+
+// Table entry of job `in` on tree slot `ti`, its tiles numbered from `*tiles`
+// (advanced past them). Sets *lod when the job is get_by_ray_at_lod.
+int fill_job(BatchJob& j, const BatchJobIn& in, uint32_t ti, uint64_t* tiles, bool* lod,
+             std::string* err) {
+  std::memset(&j, 0, sizeof(j));
+  j.tree = ti;
+  j.kind = in.kind;
+  j.width = in.width;
+  j.height = in.kind == 0 ? in.height : 1u;
+  if (in.kind == 0) {
+    j.tiles_x = (in.width + 7) / 8;
+    const uint64_t image_tiles = static_cast<uint64_t>(j.tiles_x) * ((in.height + 7) / 8);
+    const uint64_t limit = in.tile_limit ? in.tile_limit : image_tiles - std::min<uint64_t>(
+                                                                in.first_tile, image_tiles);
+    if (image_tiles >= (1ull << 26) || in.first_tile + limit > image_tiles) {
+      *err = "render job tile range outside the image, or image above 2^32 pixels";
+      return SVO_INVALID_ARGUMENT;
+    }
+    j.tile_count = static_cast<uint32_t>(limit);
+    j.first_tile = in.first_tile;
+    j.tile_major = in.tile_major ? 1u : 0u;
+    j.cam = in.cam;
+  } else {
+    if (in.width && !in.rays) {
+      *err = "cast job without rays";
+      return SVO_INVALID_ARGUMENT;
+    }
+    j.tiles_x = 0;
+    j.tile_count = (in.width + 63) / 64;
+    j.rays = in.rays;
+  }
+  j.tile_begin = static_cast<uint32_t>(*tiles);
+  j.out = to_ptrs(in.out);
+  if (!(in.viewing_distance >= 0.0f)) {
+    *err = "viewing distance must be >= 0 (0: get_by_ray)";
+    return SVO_INVALID_ARGUMENT;
+  }
+  j.viewing_distance = in.viewing_distance;
+  *lod = *lod || in.viewing_distance > 0.0f;
+  *tiles += j.tile_count;
+  if (*tiles >= (1ull << 26)) {
+    *err = "too many tiles in one batch (at most 2^26 tiles of 64 rays)";
+    return SVO_INVALID_ARGUMENT;
+  }
+  return SVO_OK;
+}
+
+void fill_header(char* base, uint32_t n, uint64_t tiles, size_t trees_offset) {
+  BatchHeader* hdr = reinterpret_cast<BatchHeader*>(base);
+  hdr->njobs = n;
+  hdr->total_tiles = static_cast<uint32_t>(tiles);
+  hdr->chunk_tiles = static_cast<uint32_t>((tiles + kChunks - 1) / kChunks);
+  hdr->trees_offset = static_cast<uint32_t>(trees_offset);
 }
 
 }  // namespace
@@ -395,7 +450,6 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     b->pending[slot] = false;
   }
   char* base = static_cast<char*>(b->host[slot]);
-  BatchHeader* hdr = reinterpret_cast<BatchHeader*>(base);
   BatchJob* out = reinterpret_cast<BatchJob*>(base + kJobsOffset);
   const size_t trees_offset = trees_offset_for(n);
   TreeView* trees = reinterpret_cast<TreeView*>(base + trees_offset);
@@ -417,52 +471,10 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     } else {
       ti = it->second;
     }
-    BatchJob& j = out[i];
-    std::memset(&j, 0, sizeof(j));
-    j.tree = ti;
-    j.kind = in.kind;
-    j.width = in.width;
-    j.height = in.kind == 0 ? in.height : 1u;
-    if (in.kind == 0) {
-      j.tiles_x = (in.width + 7) / 8;
-      const uint64_t image_tiles = static_cast<uint64_t>(j.tiles_x) * ((in.height + 7) / 8);
-      const uint64_t limit = in.tile_limit ? in.tile_limit : image_tiles - std::min<uint64_t>(
-                                                                  in.first_tile, image_tiles);
-      if (image_tiles >= (1ull << 26) || in.first_tile + limit > image_tiles) {
-        *err = "render job tile range outside the image, or image above 2^32 pixels";
-        return SVO_INVALID_ARGUMENT;
-      }
-      j.tile_count = static_cast<uint32_t>(limit);
-      j.first_tile = in.first_tile;
-      j.tile_major = in.tile_major ? 1u : 0u;
-      j.cam = in.cam;
-    } else {
-      if (in.width && !in.rays) {
-        *err = "cast job without rays";
-        return SVO_INVALID_ARGUMENT;
-      }
-      j.tiles_x = 0;
-      j.tile_count = (in.width + 63) / 64;
-      j.rays = in.rays;
-    }
-    j.tile_begin = static_cast<uint32_t>(tiles);
-    j.out = to_ptrs(in.out);
-    if (!(in.viewing_distance >= 0.0f)) {
-      *err = "viewing distance must be >= 0 (0: get_by_ray)";
-      return SVO_INVALID_ARGUMENT;
-    }
-    j.viewing_distance = in.viewing_distance;
-    any_lod = any_lod || in.viewing_distance > 0.0f;
-    tiles += j.tile_count;
-    if (tiles >= (1ull << 26)) {
-      *err = "too many tiles in one batch (at most 2^26 tiles of 64 rays)";
-      return SVO_INVALID_ARGUMENT;
-    }
+    const int st = fill_job(out[i], in, ti, &tiles, &any_lod, err);
+    if (st != SVO_OK) return st;
   }
-  hdr->njobs = n;
-  hdr->total_tiles = static_cast<uint32_t>(tiles);
-  hdr->chunk_tiles = static_cast<uint32_t>((tiles + kChunks - 1) / kChunks);
-  hdr->trees_offset = static_cast<uint32_t>(trees_offset);
+  fill_header(base, n, tiles, trees_offset);
   b->last_tiles = tiles;
   const size_t used = trees_offset + tree_index.size() * sizeof(TreeView);
   if (any_lod && !b->exec_lod) {
