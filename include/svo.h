@@ -387,6 +387,18 @@ void svo_batch_destroy(svo_batch* batch);
 int svo_batch_run(svo_batch* batch, const svo_job* jobs, uint32_t n, void* stream);
 /* Total 64-ray tiles of the last run. */
 uint64_t svo_batch_tiles(const svo_batch* batch);
+/* svo_batch_run without a context and without allocating: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * job table (at most 256 jobs) and the work counters live in a fixed device
+ * array of the library (about 64 KiB per device, held until the process
+ * ends) that every call on a device shares: calls that overlap on one device
+ * race, so the caller orders them (one stream, or events). The table reaches
+ * the device through kernel arguments, so the call allocates nothing, copies
+ * from no pageable memory and launches on `stream` asynchronously; it uses
+ * the kernel variant currently selected (svo_select_batch_variant). More than
+ * 256 jobs is SVO_INVALID_ARGUMENT. Results equal svo_batch_run's. */
+int svo_batch_run_static(const svo_job* jobs, uint32_t n, void* stream);
 /* Kernel variants of batched launches (no counterpart in the reference):
  * "classic" (one 64-ray tile per work grab, one ray per lane) and lane-refill
  * forms (several tiles per grab; finished lanes start the next rays of the

@@ -161,6 +161,7 @@ _SIGS = {
     "svo_batch_create": (C.c_int, [_U32, C.POINTER(_P)]),
     "svo_batch_destroy": (None, [_P]),
     "svo_batch_run": (C.c_int, [_P, _P, _U32, _P]),
+    "svo_batch_run_static": (C.c_int, [_P, _U32, _P]),
     "svo_batch_tiles": (C.c_uint64, [_P]),
     "svo_batch_variant_count": (C.c_int, []),
     "svo_batch_variant_name": (C.c_char_p, [C.c_int]),

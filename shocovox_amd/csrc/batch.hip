@@ -30,8 +30,11 @@
 #include <unordered_map>
 #include <vector>
 
+#include <mutex>
+
 #include "batch.hpp"
 #include "device_common.hpp"
+#include "handles.hpp"
 #include "refill.hpp"
 
 namespace svo {
@@ -500,4 +503,165 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
   return SVO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched runs that allocate nothing (svo_batch_run_static). The job table
+// and the work counters live in `scratch`, a fixed array in device memory
+// that every call on a device shares; the table reaches it through the
+// arguments of small upload kernels, so a call makes no allocation, no copy
+// from pageable memory and no graph.
+
+namespace {
+constexpr uint32_t kStaticMaxJobs = 256;
+constexpr size_t kStaticTableBytes =
+    trees_offset_for(kStaticMaxJobs) + kStaticMaxJobs * sizeof(TreeView);
+constexpr size_t kStaticCountersOffset = (kStaticTableBytes + 255) & ~size_t(255);
+constexpr size_t kScratchBytes = kStaticCountersOffset + kChunks * sizeof(Counter);
+}  // namespace
+
+// Job table [0, kStaticTableBytes) and XCD work counters of
+// svo_batch_run_static; each device holds its own copy for the life of the
+// process. Host code never passes this name to a kernel or a copy (on the
+// host it does not denote device memory): it reads the device's address of
+// the array once per device with hipGetSymbolAddress.
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+
+// One upload kernel carries up to 3 KiB of the table in its arguments
+// (kernel arguments are limited to 4 KiB).
+constexpr uint32_t kUploadWords = 768;
+struct UploadChunk {
+  uint32_t offset_words;
+  uint32_t count;
+  uint32_t words[kUploadWords];
+};
+
+__global__ __launch_bounds__(256) void upload_table_kernel(uint32_t* __restrict__ dst,
+                                                           const UploadChunk c) {
+  for (uint32_t i = threadIdx.x; i < c.count; i += 256u) dst[c.offset_words + i] = c.words[i];
+}
+
+constexpr int kStaticMaxDevices = 64;
+struct StaticDevice {
+  char* scratch = nullptr;  // the device's address of `scratch`
+  uint32_t cus = 0;
+};
+std::mutex g_static_mu;  // guards g_static and the host image of the table
+StaticDevice g_static[kStaticMaxDevices];
+alignas(64) char g_static_table[kStaticTableBytes];
+
+// Address of `scratch` on `device` (current), read once per device.
+int static_device(int device, StaticDevice** out, std::string* err) {
+  StaticDevice& d = g_static[device];
+  if (!d.scratch) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", err);
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute", err);
+    d.cus = static_cast<uint32_t>(std::max(cus, 1));
+    d.scratch = static_cast<char*>(p);
+  }
+  *out = &d;
+  return SVO_OK;
+}
+
+int run_static(const svo_job* jobs, uint32_t n, void* stream, std::string* err) {
+  if (n > kStaticMaxJobs) {
+    *err = "svo_batch_run_static takes at most " + std::to_string(kStaticMaxJobs) +
+           " jobs (its job table has a fixed size); use svo_batch_run for more";
+    return SVO_INVALID_ARGUMENT;
+  }
+  if (n && !jobs) {
+    *err = "NULL argument";
+    return SVO_INVALID_ARGUMENT;
+  }
+  int device = -1;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice", err);
+  if (device < 0 || device >= kStaticMaxDevices) {
+    *err = "device index beyond the 64 devices svo_batch_run_static supports";
+    return SVO_INVALID_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> g(g_static_mu);
+  StaticDevice* sd = nullptr;
+  int st = static_device(device, &sd, err);
+  if (st != SVO_OK) return st;
+  char* base = g_static_table;
+  BatchJob* out = reinterpret_cast<BatchJob*>(base + kJobsOffset);
+  const size_t trees_offset = trees_offset_for(n);
+  TreeView* trees = reinterpret_cast<TreeView*>(base + trees_offset);
+  const DeviceTree* keys[kStaticMaxJobs];
+  uint32_t ntrees = 0;
+  uint64_t tiles = 0;
+  bool any_lod = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    const svo_job& j = jobs[i];
+    if (!j.tree || !j.tree->dev || j.tree->dev->device != device) {
+      *err = "job tree is missing or lives on another device";
+      return SVO_INVALID_ARGUMENT;
+    }
+    if (j.kind > 1) {
+      *err = "unknown job kind";
+      return SVO_INVALID_ARGUMENT;
+    }
+    BatchJobIn in;
+    in.tree = j.tree->dev;
+    in.kind = j.kind;
+    in.width = j.width;
+    in.height = j.height;
+    in.rays = j.rays;
+    in.out = j.out;
+    in.viewing_distance = j.viewing_distance;
+    in.cam = Camera{};
+    if (j.kind == 0)
+      in.cam = make_camera(V3f{j.viewport.origin[0], j.viewport.origin[1], j.viewport.origin[2]},
+                           V3f{j.viewport.direction[0], j.viewport.direction[1],
+                               j.viewport.direction[2]},
+                           j.viewport.frustum[0], j.viewport.frustum[1], j.viewport.fov, j.width,
+                           j.height);
+    uint32_t ti = 0;
+    while (ti < ntrees && keys[ti] != in.tree) ++ti;
+    if (ti == ntrees) {
+      keys[ntrees++] = in.tree;
+      trees[ti] = in.tree->view;
+    }
+    st = fill_job(out[i], in, ti, &tiles, &any_lod, err);
+    if (st != SVO_OK) return st;
+  }
+  if (tiles == 0) return SVO_OK;  // nothing to cast, nothing to write
+  fill_header(base, n, tiles, trees_offset);
+  const size_t used = trees_offset + ntrees * sizeof(TreeView);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(sd->scratch);
+  const uint32_t words = static_cast<uint32_t>((used + 3) / 4);
+  for (uint32_t w0 = 0; w0 < words; w0 += kUploadWords) {
+    UploadChunk c;
+    c.offset_words = w0;
+    c.count = std::min(kUploadWords, words - w0);
+    std::memcpy(c.words, base + static_cast<size_t>(w0) * 4, static_cast<size_t>(c.count) * 4);
+    hipLaunchKernelGGL(upload_table_kernel, dim3(1), dim3(256), 0, s, dst, c);
+  }
+  Counter* counters = reinterpret_cast<Counter*>(sd->scratch + kStaticCountersOffset);
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters);
+  const BatchVariant& v = kBatchVariants[g_batch_variant];
+  const BatchFn fn = any_lod ? kBatchLodKernel : v.fn;
+  const uint32_t grid = sd->cus * v.groups_per_cu;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, s, static_cast<const char*>(sd->scratch),
+                     counters);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "svo_batch_run_static launch", err);
+  return SVO_OK;
+}
+
+}  // namespace
+
 }  // namespace svo
+
+extern "C" int svo_batch_run_static(const svo_job* jobs, uint32_t n, void* stream) {
+  std::string err;
+  const int st = svo::run_static(jobs, n, stream, &err);
+  if (st != SVO_OK) svo::set_last_error(err);
+  return st;
+}
