@@ -38,15 +38,15 @@ __device__ inline void store_result(const TreeView& t, const HitPtrs& o, uint64_
     o.albedo[i] = a;
   }
   if (o.voxel) {
+    uint32_t vx = EMPTY_MARKER, vy = EMPTY_MARKER, vz = EMPTY_MARKER;
     if (r.hit) {
-      o.voxel[3 * i + 0] = hit_voxel_axis(r.impact.x, r.cell.min.x, r.cell.size);
-      o.voxel[3 * i + 1] = hit_voxel_axis(r.impact.y, r.cell.min.y, r.cell.size);
-      o.voxel[3 * i + 2] = hit_voxel_axis(r.impact.z, r.cell.min.z, r.cell.size);
-    } else {
-      o.voxel[3 * i + 0] = EMPTY_MARKER;
-      o.voxel[3 * i + 1] = EMPTY_MARKER;
-      o.voxel[3 * i + 2] = EMPTY_MARKER;
+      vx = hit_voxel_axis(r.impact.x, r.cell.min.x, r.cell.size);
+      vy = hit_voxel_axis(r.impact.y, r.cell.min.y, r.cell.size);
+      vz = hit_voxel_axis(r.impact.z, r.cell.min.z, r.cell.size);
     }
+    o.voxel[3 * i + 0] = vx;
+    o.voxel[3 * i + 1] = vy;
+    o.voxel[3 * i + 2] = vz;
   }
   if (o.distance) o.distance[i] = r.hit ? length(r.impact - org) : __builtin_inff();
   if (o.impact) {

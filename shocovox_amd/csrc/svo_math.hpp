@@ -200,13 +200,18 @@ SVO_HD inline V3f dda_scale_factors(V3f d) {
              sqrtf((xz * xz + 1.0f) + yz * yz)};
 }
 
+// c - s * d for a signum s (+1, -1 or NaN): s * d is d negated or not (exact)
+// or NaN, so a fused multiply-add, which rounds once, gives exactly the
+// reference's product-then-difference (raytracing_on_cpu.rs:136-140).
+SVO_HD inline float sub_signed(float c, float s, float d) { return fmaf(-s, d, c); }
+
 // dda_step_to_next_sibling (raytracing_on_cpu.rs:124-152); `sig` is the
 // per-ray signum of the direction, hoisted because it never changes.
 SVO_HD inline V3f dda_step(V3f dir, V3f sig, V3f* p, Cube b, V3f scale) {
   const V3f diff = *p - b.min;
-  const float snx = b.size * rmax(sig.x, 0.0f) - sig.x * diff.x;
-  const float sny = b.size * rmax(sig.y, 0.0f) - sig.y * diff.y;
-  const float snz = b.size * rmax(sig.z, 0.0f) - sig.z * diff.z;
+  const float snx = sub_signed(b.size * rmax(sig.x, 0.0f), sig.x, diff.x);
+  const float sny = sub_signed(b.size * rmax(sig.y, 0.0f), sig.y, diff.y);
+  const float snz = sub_signed(b.size * rmax(sig.z, 0.0f), sig.z, diff.z);
   const float dx = fabsf(snx * scale.x);
   const float dy = fabsf(sny * scale.y);
   const float dz = fabsf(snz * scale.z);
@@ -219,12 +224,12 @@ SVO_HD inline V3f dda_step(V3f dir, V3f sig, V3f* p, Cube b, V3f scale) {
 
 // dda_step reporting which axes stepped (m == d{x,y,z}, the axes whose step
 // component is signum(direction)); the others' components are +0.0.
-SVO_HD inline void dda_step_axes(V3f dir, V3f sig, V3f* p, Cube b, V3f scale, bool* ax, bool* ay,
-                                 bool* az) {
-  const V3f diff = *p - b.min;
-  const float snx = b.size * rmax(sig.x, 0.0f) - sig.x * diff.x;
-  const float sny = b.size * rmax(sig.y, 0.0f) - sig.y * diff.y;
-  const float snz = b.size * rmax(sig.z, 0.0f) - sig.z * diff.z;
+SVO_HD inline void dda_step_axes(V3f dir, V3f sig, V3f* p, V3f bmin, float size, V3f scale,
+                                 bool* ax, bool* ay, bool* az) {
+  const V3f diff = *p - bmin;
+  const float snx = sub_signed(size * rmax(sig.x, 0.0f), sig.x, diff.x);
+  const float sny = sub_signed(size * rmax(sig.y, 0.0f), sig.y, diff.y);
+  const float snz = sub_signed(size * rmax(sig.z, 0.0f), sig.z, diff.z);
   const float dx = fabsf(snx * scale.x);
   const float dy = fabsf(sny * scale.y);
   const float dz = fabsf(snz * scale.z);

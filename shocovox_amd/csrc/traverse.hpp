@@ -200,48 +200,65 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   int ix = clampi(f2i32(pib.x));
   int iy = clampi(f2i32(pib.y));
   int iz = clampi(f2i32(pib.z));
-  int flat = ix + iy * dim + iz * dim * dim;
   const float unit = div_size<P2>(bb.size, fdim, t.inv_dim);
-  Cube cb{bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
-                       unit,
-          unit};
+  V3f cmin = bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
+                          unit;
   const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
   // A step moves the cell by signum(direction) on the axes that stepped and
   // leaves the others (their step component is +0.0, and cell corners are
   // never -0.0): the per-axis moves are fixed for the whole probe.
   const V3f umove{sig.x * unit, sig.y * unit, sig.z * unit};
   const int mx = step_int(sig.x), my = step_int(sig.y), mz = step_int(sig.z);
-  for (int guard = 0; guard < 4 * dim + 16; ++guard) {
+  // The loop has one exit: the cell is outside the brick, or occupied, or
+  // the step bound is reached (then `in` still holds and `occupied` not).
+  // The occupancy word is read for every cell, from index 0 when the cell
+  // is outside, so that the test needs no branch.
+  const uint32_t shift = static_cast<uint32_t>(__builtin_ctz(t.dim));
+  // ix + iy dim + iz dim^2 (P2: the factors as shifts)
+  auto flat_of = [&](int x, int y, int z) -> uint32_t {
+    if constexpr (P2) {
+      return static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << shift) |
+             (static_cast<uint32_t>(z) << (2u * shift));
+    } else {
+      return static_cast<uint32_t>(x + y * dim + z * dim * dim);
+    }
+  };
+  bool in, occupied;
+  for (int left = 4 * dim + 16;;) {
     if constexpr (P2) {
       // with dim a power of two, all three are in [0, dim) exactly when
       // their bitwise or is (a negative index has the top bit set): one
       // test instead of six
-      if (static_cast<uint32_t>(ix | iy | iz) >= static_cast<uint32_t>(dim)) return false;
+      in = static_cast<uint32_t>(ix | iy | iz) < static_cast<uint32_t>(dim);
     } else {
-      if (ix < 0 || ix >= dim || iy < 0 || iy >= dim || iz < 0 || iz >= dim) return false;
+      in = ix >= 0 && ix < dim && iy >= 0 && iy < dim && iz >= 0 && iz < dim;
     }
-    flat = ix + iy * dim + iz * dim * dim;
-    if ((occ[static_cast<uint32_t>(flat) >> 5] >> (static_cast<uint32_t>(flat) & 31u)) & 1u) {
-      const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
-      r->pix = t.brick_pix[static_cast<uint64_t>(word) * n3 + static_cast<uint32_t>(flat)];
-      const V3f scaled = V3f{static_cast<float>(ix), static_cast<float>(iy),
-                             static_cast<float>(iz)} *
-                         bb.size;
-      const Cube hb{bb.min + div3_size<P2>(scaled, fdim, t.inv_dim),
-                    div_size<P2>(bb.size, fdim, t.inv_dim)};
-      r->impact = *p;
-      r->cell = hb;
-      return true;
-    }
+    const uint32_t flat = in ? flat_of(ix, iy, iz) : 0u;
+    const uint32_t bits = occ[flat >> 5];
+    occupied = in & (((bits >> (flat & 31u)) & 1u) != 0u);
+    if (!in | occupied) break;
     bool sx, sy, sz;
-    dda_step_axes(d, sig, p, cb, scale, &sx, &sy, &sz);
-    cb.min = V3f{sx ? cb.min.x + umove.x : cb.min.x, sy ? cb.min.y + umove.y : cb.min.y,
-                 sz ? cb.min.z + umove.z : cb.min.z};
+    dda_step_axes(d, sig, p, cmin, unit, scale, &sx, &sy, &sz);
+    cmin = V3f{sx ? cmin.x + umove.x : cmin.x, sy ? cmin.y + umove.y : cmin.y,
+               sz ? cmin.z + umove.z : cmin.z};
     ix += sx ? mx : 0;
     iy += sy ? my : 0;
     iz += sz ? mz : 0;
+    if (--left == 0) break;
   }
-  r->flags |= FLAG_ITERATION_GUARD;
+  if (occupied) {
+    const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
+    r->pix = t.brick_pix[static_cast<uint64_t>(word) * n3 + flat_of(ix, iy, iz)];
+    const V3f scaled = V3f{static_cast<float>(ix), static_cast<float>(iy),
+                           static_cast<float>(iz)} *
+                       bb.size;
+    const Cube hb{bb.min + div3_size<P2>(scaled, fdim, t.inv_dim),
+                  div_size<P2>(bb.size, fdim, t.inv_dim)};
+    r->impact = *p;
+    r->cell = hb;
+    return true;
+  }
+  if (in) r->flags |= FLAG_ITERATION_GUARD;
   return false;
 }
 
