@@ -173,7 +173,8 @@ def batched_bench(args, render_variant: str) -> dict:
     import torch
 
     from shocovox_amd import (Batch, DeviceTree, Octree, alloc_hits, batch_variants,
-                              select_batch_variant, workloads as W)
+                              render_variants, select_batch_variant, select_render_variant,
+                              workloads as W)
 
     t0 = time.time()
     specs, views = W.batched_pool_specs(args.seed, args.batched_trees, args.batched_max_size)
@@ -200,13 +201,29 @@ def batched_bench(args, render_variant: str) -> dict:
         for (ti, vp, w, h), b in zip(views, plain_bufs):
             dtrees[ti].render_into(vp, w, h, b, stream)
 
-    plain_pass()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(reps):
+    # the per-viewport baseline runs the render variant fastest on this pool,
+    # among the headline's pick and the two forms of the one-tile kernel
+    rnames = render_variants()
+    cands = []
+    for name in (render_variant, "w5-glut-8x8", "w5-glut-8x8-ww-g1"):
+        if name in rnames and name not in cands:
+            cands.append(name)
+    plain_ms = {}
+    for name in cands:
+        select_render_variant(rnames.index(name))
         plain_pass()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            plain_pass()
+        torch.cuda.synchronize()
+        plain_ms[name] = (time.perf_counter() - t1) / reps * 1e3
+        log(f"batched pool, per-viewport launches with {name}: {plain_ms[name]:.3f} ms")
+    plain_variant = min(plain_ms, key=plain_ms.get)
+    select_render_variant(rnames.index(plain_variant))
+    plain_pass()  # the reference outputs for the equality check
     torch.cuda.synchronize()
-    plain_s = (time.perf_counter() - t1) / reps
+    plain_s = plain_ms[plain_variant] / 1e3
     out = {"workload": (f"{len(trees)} trees (sizes {min(sizes)}..{max(sizes)} log-uniform, "
                         f"{sizes.count(1024)} of 1024^3; brick dims 4/8/16; {kinds[2]} caves, "
                         f"{kinds[4]} heightfield; {pool_bytes / 2**30:.2f} GiB resident), "
@@ -214,7 +231,8 @@ def batched_bench(args, render_variant: str) -> dict:
            "trees": len(trees), "tree_sizes": sorted(sizes), "pool_bytes": pool_bytes,
            "viewports": len(views), "rays": rays,
            "plain": {"ms": round(plain_s * 1e3, 3), "mrays_per_s": round(rays / plain_s / 1e6, 1),
-                     "launches": len(views), "render_variant": render_variant},
+                     "launches": len(views), "render_variant": plain_variant,
+                     "variants_ms": {k: round(v, 3) for k, v in plain_ms.items()}},
            "batched": {}}
     names = batch_variants()
     best = None
@@ -304,7 +322,8 @@ def served_bench(args, capacity_mrays: float) -> dict:
     equal p99."""
     import torch
 
-    from shocovox_amd import batch_variants, current_batch_variant, serve
+    from shocovox_amd import (batch_variants, current_batch_variant, current_render_variant,
+                              render_variants, serve)
 
     t0 = time.time()
     specs, pristine = served_pool(args)
@@ -422,6 +441,7 @@ def served_bench(args, capacity_mrays: float) -> dict:
         "device_total_bytes": total,
         "max_inflight": args.served_inflight, "max_batch_rays": args.served_batch_rays,
         "batch_variant": batch_variants()[current_batch_variant()],
+        "render_variant": render_variants()[current_render_variant()],
         "arrival_seconds": args.served_seconds,
         "nominal_capacity_qps": round(qps_cap, 1),
         "p99_target_ms": slo, "p99_target_from": f"plain mode's p99 at load {ref}",
@@ -435,6 +455,119 @@ def served_bench(args, capacity_mrays: float) -> dict:
         "search": {m: results[m]["search"] for m in modes},
         "warmup_session": "1 s at load 0.6 per mode, each mode on its own copy of the pool",
     }
+
+
+def lod_mode(args) -> int:
+    """get_by_ray_at_lod on the caves workload with MIP maps on: per step the
+    8 poses rendered by render_lod_kernel and 2M incoherent rays cast by
+    cast_lod_kernel, at viewing distance --lod-distance (prints one JSON line;
+    the figures of the LOD kernels' profiles)."""
+    import torch
+
+    from shocovox_amd import DeviceTree, alloc_hits, workloads as W
+
+    cfg = W.CAVES
+    t0 = time.time()
+    tree = W.build_scene(cfg, seed=args.seed)
+    tree.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    d = DeviceTree(tree)
+    vps = W.viewports(cfg, seed=args.seed)
+    inc = torch.from_numpy(W.incoherent_rays(cfg.size, 2_000_000, seed=args.seed + 4)).to("cuda:0")
+    log(f"lod: caves with MIP maps resident in {time.time() - t0:.1f}s")
+    vd = args.lod_distance
+    fields = ("hit", "albedo", "voxel", "distance")
+    outs = [alloc_hits(cfg.width * cfg.height, "cuda:0", fields) for _ in vps]
+
+    def step():
+        for vp, o in zip(vps, outs):
+            o.update(d.render_at_lod(vp, cfg.width, cfg.height, vd, fields))
+        d.cast_at_lod(inc, vd, fields)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t1) / args.steps
+    rays = cfg.width * cfg.height * len(vps) + inc.shape[0]
+    hits = float(torch.stack([o["hit"].float().mean() for o in outs]).mean())
+    print(json.dumps({"mode": "lod", "viewing_distance": vd, "rays_per_step": rays,
+                      "ms_per_step": round(el * 1e3, 3), "mrays_per_s": round(rays / el / 1e6, 1),
+                      "hit_rate": round(hits, 4)}), flush=True)
+    return 0
+
+
+def spheres_block(args) -> dict:
+    """The 2048^3 spheres workload (brick dim 32): one step is a 3840x2160
+    render plus 8M incoherent rays (origins uniform in the volume,
+    directions uniform on the sphere), in one batched launch (the fastest of
+    a few batch variants whose outputs equal the first one's) and as a render
+    launch plus a cast launch."""
+    import torch
+
+    from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variants,
+                              select_batch_variant, workloads as W)
+
+    cfg = W.SPHERES
+    t0 = time.time()
+    tree = W.build_scene(cfg, seed=args.seed)
+    d = DeviceTree(tree)
+    del tree
+    vp = W.viewports(cfg, seed=args.seed)[0]
+    inc = torch.from_numpy(W.incoherent_rays(cfg.size, 8_000_000, seed=args.seed + 6)).to("cuda:0")
+    rays = cfg.width * cfg.height + inc.shape[0]
+    log(f"spheres: built and resident ({d.device_bytes / 2**30:.2f} GiB) in {time.time() - t0:.1f}s")
+    fields = ("hit", "albedo", "voxel", "distance")
+    pr, pc = alloc_hits(cfg.width * cfg.height, "cuda:0", fields), alloc_hits(inc.shape[0],
+                                                                            "cuda:0", fields)
+    stream = torch.cuda.current_stream()
+    steps = max(3, args.steps // 4)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / steps
+
+    def launches():
+        d.render_into(vp, cfg.width, cfg.height, pr, stream)
+        d.cast_into(inc, inc.shape[0], pc, stream)
+
+    per_launch = timed(launches)
+    ref = [{k: b[k].clone() for k in fields} for b in (pr, pc)]
+    names = batch_variants()
+    tuning = {}
+    for name in ("classic-w5-g5", "ww-w5-g5", "ww-w6-g6", "classic-w5-g5-n73"):
+        if name not in names:
+            continue
+        select_batch_variant(names.index(name))
+        bt = Batch(2)
+        jobs = [Batch.render_job(d, vp, cfg.width, cfg.height, pr), Batch.cast_job(d, inc, pc)]
+        el = timed(lambda: bt.run(jobs, stream))
+        same = all(torch.equal(b[k], r[k]) for b, r in zip((pr, pc), ref) for k in fields)
+        tuning[name] = {"ms": round(el * 1e3, 3), "equal_to_launches": bool(same)}
+        log(f"spheres batched {name}: {el * 1e3:.3f} ms, equal {same}")
+        del bt
+    best = min((n for n in tuning if tuning[n]["equal_to_launches"]),
+               key=lambda n: tuning[n]["ms"], default=None)
+    out = {"workload": (f"spheres: {cfg.size}^3 octree, brick dim {cfg.brick_dim}, "
+                        f"{cfg.width}x{cfg.height} primary rays + 8M incoherent rays per step"),
+           "rays_per_step": rays, "device_bytes": d.device_bytes, "steps": steps,
+           "launches": {"ms_per_step": round(per_launch * 1e3, 3),
+                        "mrays_per_s": round(rays / per_launch / 1e6, 1)},
+           "batched": tuning, "best_variant": best}
+    if best:
+        ms = tuning[best]["ms"]
+        out["ms_per_step"] = ms
+        out["mrays_per_s"] = round(rays / (ms * 1e-3) / 1e6, 1)
+    del d
+    torch.cuda.synchronize()
+    return out
 
 
 def main() -> int:
@@ -463,6 +596,8 @@ def main() -> int:
     ap.add_argument("--batched-all-variants", action="store_true",
                     help="time the lane-refill batch variants too")
     ap.add_argument("--no-served", action="store_true", help="skip the served-mode sessions")
+    ap.add_argument("--no-spheres", action="store_true",
+                    help="skip the 2048^3 spheres block (4K + 8M incoherent rays per step)")
     ap.add_argument("--served-trees", type=int, default=32)
     ap.add_argument("--served-max-size", type=int, default=2048)
     ap.add_argument("--served-budget-gb", type=float, default=40.0,
@@ -480,9 +615,12 @@ def main() -> int:
                     help="upper end of the capacity search, as a fraction of capacity")
     ap.add_argument("--served-kernel-mode", action="store_true",
                     help="also serve with the persistent kernel for every launch")
-    ap.add_argument("--mode", default="render", choices=["render", "copy"],
+    ap.add_argument("--mode", default="render", choices=["render", "copy", "lod"],
                     help="'copy' runs a plain device-to-device copy of known size (for "
-                         "calibrating rocprofv3 byte counters)")
+                         "calibrating rocprofv3 byte counters); 'lod' times get_by_ray_at_lod "
+                         "on the caves workload with MIP maps on")
+    ap.add_argument("--lod-distance", type=float, default=64.0,
+                    help="viewing distance of --mode lod")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -495,10 +633,12 @@ def main() -> int:
 
     if args.mode == "copy":
         return copy_calibration(args)
+    if args.mode == "lod":
+        return lod_mode(args)
 
     from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variants, collect_kernel_ms,
-                              render_variants, select_batch_variant, select_render_variant,
-                              set_timing, workloads as W)
+                              current_batch_variant, render_variants, select_batch_variant,
+                              select_render_variant, set_timing, workloads as W)
 
     cfg = W.CONFIGS[args.workload]
     t0 = time.time()
@@ -718,9 +858,16 @@ def main() -> int:
     }
     if not args.no_batched:
         line["batched"] = batched_bench(args, best)
+        # the served sessions' plain launches use the render variant that
+        # was fastest on the batched workload's mix of trees
+        select_render_variant(render_variants().index(line["batched"]["plain"]["render_variant"]))
     if not args.no_served:
         # offered loads as fractions of the one-launch-per-frame capacity
         line["served"] = served_bench(args, per_frame["mrays_per_s"])
+    if not args.no_spheres and args.workload == "caves":
+        prev = current_batch_variant()
+        line["spheres"] = spheres_block(args)
+        select_batch_variant(prev)
     print(json.dumps(line), flush=True)
     return 0
 

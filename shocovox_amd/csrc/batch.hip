@@ -104,9 +104,21 @@ __device__ inline const TreeView* trees_of(const char* table) {
 // LOD compiles in get_by_ray_at_lod for jobs with a viewing distance; a
 // context launches that form only for runs that hold such jobs, so the plain
 // form keeps its smaller register allocation.
-template <int MIN_WAVES, bool WW = false, bool LOD = false>
+// Render jobs are cut into TW x TH pixel tiles (64 pixels, one per lane).
+// NODES > 0 stages the first NODES node records of a tree (the top levels of
+// the breadth-first numbering) in LDS, where the traversal reads them:
+// without SHARED each wave keeps the records of the tree of its current job
+// and reloads them when it moves to a job on another tree; with SHARED the
+// workgroup keeps those of the tree of the run's first job, loaded once at
+// the start (jobs on other trees read global memory).
+template <int MIN_WAVES, bool WW = false, bool LOD = false, int TW = 8, int TH = 8, int NODES = 0,
+          bool SHARED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters) {
+  static_assert(TW * TH == 64, "a tile is one pixel per lane");
+  static_assert(NODES == 0 || !LOD, "the LOD form reads every record from global memory");
+  constexpr int kCaches = NODES > 0 && !SHARED ? 4 : 1;
+  __shared__ NodeRec s_nodes[kCaches][NODES > 0 ? NODES : 1];
   const BatchHeader* hdr = header_of(table);
   const BatchJob* jobs = jobs_of(table);
   const TreeView* trees = trees_of(table);
@@ -115,6 +127,20 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
   const uint32_t total = hdr->total_tiles;
   const uint32_t chunk = hdr->chunk_tiles;
   const uint32_t home = blockIdx.x % kChunks;
+  NodeRec* cache = s_nodes[kCaches > 1 ? (threadIdx.x >> 6) : 0];
+  uint32_t cached_tree = 0xFFFFFFFFu;
+  uint32_t ncached = 0;
+  if constexpr (NODES > 0 && SHARED) {
+    if (njobs > 0) {
+      const TreeView& tv0 = trees[jobs[0].tree];
+      ncached = min(static_cast<uint32_t>(NODES), tv0.node_count);
+      const uint4* src = reinterpret_cast<const uint4*>(tv0.nodes);
+      uint4* dst = reinterpret_cast<uint4*>(cache);
+      for (uint32_t i = threadIdx.x; i < ncached * 4u; i += 256u) dst[i] = src[i];
+      cached_tree = jobs[0].tree;
+    }
+    __syncthreads();
+  }
   for (uint32_t k = 0; k < kChunks; ++k) {
     const uint32_t c = (home + k) % kChunks;
     const uint32_t begin = c * chunk;
@@ -131,14 +157,29 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
       const BatchJob& job = jobs[j];
       const TreeView& tv = trees[job.tree];
       const uint32_t local = tile - job.tile_begin;
+      if constexpr (NODES > 0 && !SHARED) {
+        if (job.tree != cached_tree) {
+          // the wave's own cache: its lanes write it and then read what the
+          // other lanes wrote, so the writes complete before the reads start
+          ncached = min(static_cast<uint32_t>(NODES), tv.node_count);
+          const uint4* src = reinterpret_cast<const uint4*>(tv.nodes);
+          uint4* dst = reinterpret_cast<uint4*>(cache);
+          for (uint32_t i = lane; i < ncached * 4u; i += 64u) dst[i] = src[i];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          cached_tree = job.tree;
+        }
+      }
+      const uint32_t ntop = NODES > 0 && job.tree == cached_tree ? ncached : 0u;
       // the lane's ray; one inlined traversal serves both job kinds
       V3f o, d;
       uint32_t idx;
       bool valid;
       if (job.kind == 0) {
         const uint32_t image_tile = local + job.first_tile;
-        const uint32_t x = (image_tile % job.tiles_x) * 8u + (lane & 7u);
-        const uint32_t y = (image_tile / job.tiles_x) * 8u + (lane >> 3);
+        const uint32_t x = (image_tile % job.tiles_x) * TW + (lane % TW);
+        const uint32_t y = (image_tile / job.tiles_x) * TH + (lane / TW);
         valid = x < job.width && y < job.height;
         o = job.cam.origin;
         d = valid ? camera_ray_dir(job.cam, x, y) : V3f{0.0f, 0.0f, 0.0f};
@@ -157,10 +198,13 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
             r = cast_ray_lod<true>(tv, tv.ray_lut, o, d, job.viewing_distance);
           else
             r = cast_ray<true>(tv, o, d);
-        } else if constexpr (WW)
-          r = cast_ray_ww<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(nullptr), 0u);
-        else
-          r = cast_ray<true>(tv, o, d);
+        } else if constexpr (WW) {
+          // (the crawl shortcut does not fit the 80 VGPRs of six waves per SIMD)
+          r = cast_ray_ww<true, uint64_t, (MIN_WAVES < 6)>(
+              tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
+        } else {
+          r = cast_ray<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
+        }
         store_result(tv, job.out, idx, r, o);
       }
     }
@@ -216,22 +260,36 @@ using BatchFn = void (*)(const char*, Counter*);
 struct BatchVariant {
   const char* name;
   BatchFn fn;
+  BatchFn lod_fn;          // the form runs with get_by_ray_at_lod jobs use
   uint32_t groups_per_cu;  // 256-lane workgroups per CU in the persistent grid
+  uint32_t tile_w, tile_h; // pixel tile of render jobs
 };
 
+// runs that hold get_by_ray_at_lod jobs use the LOD form of the variant's
+// tile shape
+constexpr BatchFn kLod8x8 = batch_kernel<1, false, true>;
+constexpr BatchFn kLod16x4 = batch_kernel<1, false, true, 16, 4>;
+constexpr BatchFn kLod4x16 = batch_kernel<1, false, true, 4, 16>;
+
 const BatchVariant kBatchVariants[] = {
-    {"classic-w1", batch_kernel<1>, 4},
-    {"classic-w5", batch_kernel<5>, 4},
-    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>, 4},
-    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>, 4},
-    {"classic-w5-g5", batch_kernel<5>, 5},
-    {"ww-w5", batch_kernel<5, true>, 4},
-    {"ww-w6-g6", batch_kernel<6, true>, 6},
-    {"ww-w5-g5", batch_kernel<5, true>, 5},
+    {"classic-w1", batch_kernel<1>, kLod8x8, 4, 8, 8},
+    {"classic-w5", batch_kernel<5>, kLod8x8, 4, 8, 8},
+    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>, kLod8x8, 4, 8, 8},
+    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>, kLod8x8, 4, 8, 8},
+    {"classic-w5-g5", batch_kernel<5>, kLod8x8, 5, 8, 8},
+    {"ww-w5", batch_kernel<5, true>, kLod8x8, 4, 8, 8},
+    {"ww-w6-g6", batch_kernel<6, true>, kLod8x8, 6, 8, 8},
+    {"ww-w5-g5", batch_kernel<5, true>, kLod8x8, 5, 8, 8},
+    // tile shape
+    {"classic-w5-g5-16x4", batch_kernel<5, false, false, 16, 4>, kLod16x4, 5, 16, 4},
+    {"classic-w5-g5-4x16", batch_kernel<5, false, false, 4, 16>, kLod4x16, 5, 4, 16},
+    // top node records in LDS: 73 (three levels) per wave, or 585 (four
+    // levels) per workgroup (37 KiB: four workgroups per CU)
+    {"classic-w5-g5-n73", batch_kernel<5, false, false, 8, 8, 73>, kLod8x8, 5, 8, 8},
+    {"classic-w4-g4-s585", batch_kernel<4, false, false, 8, 8, 585, true>, kLod8x8, 4, 8, 8},
+    {"ww-w6-g6-n73", batch_kernel<6, true, false, 8, 8, 73>, kLod8x8, 6, 8, 8},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
-// runs that hold get_by_ray_at_lod jobs use this form in every variant
-constexpr BatchFn kBatchLodKernel = batch_kernel<1, false, true>;
 int g_batch_variant = 0;
 
 int hip_fail(hipError_t e, const char* what, std::string* err) {
@@ -245,16 +303,20 @@ constexpr size_t trees_offset_for(uint32_t njobs) {
 
 // Table entry of job `in` on tree slot `ti`, its tiles numbered from `*tiles`
 // (advanced past them). Sets *lod when the job is get_by_ray_at_lod.
-int fill_job(BatchJob& j, const BatchJobIn& in, uint32_t ti, uint64_t* tiles, bool* lod,
-             std::string* err) {
+int fill_job(BatchJob& j, const BatchJobIn& in, uint32_t ti, uint32_t tw, uint32_t th,
+             uint64_t* tiles, bool* lod, std::string* err) {
   std::memset(&j, 0, sizeof(j));
   j.tree = ti;
   j.kind = in.kind;
   j.width = in.width;
   j.height = in.kind == 0 ? in.height : 1u;
   if (in.kind == 0) {
-    j.tiles_x = (in.width + 7) / 8;
-    const uint64_t image_tiles = static_cast<uint64_t>(j.tiles_x) * ((in.height + 7) / 8);
+    if (in.tile_major && (tw != 8 || th != 8)) {
+      *err = "tile-major render jobs need a variant of 8x8 tiles";
+      return SVO_INVALID_ARGUMENT;
+    }
+    j.tiles_x = (in.width + tw - 1) / tw;
+    const uint64_t image_tiles = static_cast<uint64_t>(j.tiles_x) * ((in.height + th - 1) / th);
     const uint64_t limit = in.tile_limit ? in.tile_limit : image_tiles - std::min<uint64_t>(
                                                                 in.first_tile, image_tiles);
     if (image_tiles >= (1ull << 26) || in.first_tile + limit > image_tiles) {
@@ -379,14 +441,30 @@ bool batch_variant_select(int i) {
 int batch_variant_current() { return g_batch_variant; }
 
 int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err) {
+  return batch_create_variant(max_jobs, g_batch_variant, out, err);
+}
+
+int batch_variant_serving() {
+  const BatchVariant& v = kBatchVariants[g_batch_variant];
+  if (v.tile_w == 8 && v.tile_h == 8) return g_batch_variant;
+  for (int i = 0; i < kBatchVariantCount; ++i)
+    if (std::strcmp(kBatchVariants[i].name, "classic-w5-g5") == 0) return i;
+  return 0;
+}
+
+int batch_create_variant(uint32_t max_jobs, int variant, BatchContext** out, std::string* err) {
   *out = nullptr;
+  if (variant < 0 || variant >= kBatchVariantCount) {
+    *err = "no such batch variant";
+    return SVO_INVALID_ARGUMENT;
+  }
   if (max_jobs == 0 || max_jobs > (1u << 20)) {
     *err = "max_jobs must be in [1, 2^20]";
     return SVO_INVALID_ARGUMENT;
   }
   BatchContext* b = new BatchContext();
   b->max_jobs = max_jobs;
-  b->variant = g_batch_variant;
+  b->variant = variant;
   b->table_bytes = trees_offset_for(max_jobs) + max_jobs * sizeof(TreeView);
   hipError_t e = hipGetDevice(&b->device);
   int cus = 0;
@@ -474,15 +552,16 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
     } else {
       ti = it->second;
     }
-    const int st = fill_job(out[i], in, ti, &tiles, &any_lod, err);
+    const int st = fill_job(out[i], in, ti, kBatchVariants[b->variant].tile_w,
+                            kBatchVariants[b->variant].tile_h, &tiles, &any_lod, err);
     if (st != SVO_OK) return st;
   }
   fill_header(base, n, tiles, trees_offset);
   b->last_tiles = tiles;
   const size_t used = trees_offset + tree_index.size() * sizeof(TreeView);
   if (any_lod && !b->exec_lod) {
-    const int st = build_graph(b, kBatchLodKernel, &b->graph_lod, &b->copy_node_lod,
-                               &b->exec_lod, err);
+    const int st = build_graph(b, kBatchVariants[b->variant].lod_fn, &b->graph_lod,
+                               &b->copy_node_lod, &b->exec_lod, err);
     if (st != SVO_OK) return st;
   }
   hipGraphExec_t exec = any_lod ? b->exec_lod : b->exec;
@@ -627,7 +706,8 @@ int run_static(const svo_job* jobs, uint32_t n, void* stream, std::string* err) 
       keys[ntrees++] = in.tree;
       trees[ti] = in.tree->view;
     }
-    st = fill_job(out[i], in, ti, &tiles, &any_lod, err);
+    st = fill_job(out[i], in, ti, kBatchVariants[g_batch_variant].tile_w,
+                  kBatchVariants[g_batch_variant].tile_h, &tiles, &any_lod, err);
     if (st != SVO_OK) return st;
   }
   if (tiles == 0) return SVO_OK;  // nothing to cast, nothing to write
@@ -646,7 +726,7 @@ int run_static(const svo_job* jobs, uint32_t n, void* stream, std::string* err) 
   Counter* counters = reinterpret_cast<Counter*>(sd->scratch + kStaticCountersOffset);
   hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters);
   const BatchVariant& v = kBatchVariants[g_batch_variant];
-  const BatchFn fn = any_lod ? kBatchLodKernel : v.fn;
+  const BatchFn fn = any_lod ? v.lod_fn : v.fn;
   const uint32_t grid = sd->cus * v.groups_per_cu;
   hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, s, static_cast<const char*>(sd->scratch),
                      counters);

@@ -31,6 +31,12 @@ struct BatchJobIn {
 };
 
 int batch_create(uint32_t max_jobs, BatchContext** out, std::string* err);
+// A context of the given variant instead of the selected one.
+int batch_create_variant(uint32_t max_jobs, int variant, BatchContext** out, std::string* err);
+// The variant a serving loop uses: the selected one when its tiles are 8x8
+// (the tile-major chunks of the loop are laid out in 8x8 tiles), otherwise
+// classic-w5-g5.
+int batch_variant_serving();
 void batch_destroy(BatchContext* b);
 int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream, std::string* err);
 uint64_t batch_tiles(const BatchContext* b);

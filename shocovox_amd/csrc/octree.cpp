@@ -283,16 +283,6 @@ void Octree::assign_model(std::vector<NodeContent> items, std::vector<uint8_t> r
   ++edit_epoch_;
 }
 
-// pix_points_to_empty (node.rs:405-427)
-bool Octree::pix_points_to_empty(uint32_t pix) const {
-  const uint32_t c = pix_color_index(pix), d = pix_data_index(pix);
-  const bool color_empty =
-      c >= NONE16 || (c < color_palette_.size() ? (color_palette_[c] & 0xFFu) == 0 : true);
-  const bool data_empty =
-      d == NONE16 || (d < data_palette_.size() ? data_palette_[d] == 0 : true);
-  return color_empty && data_empty;
-}
-
 // pix_get_ref (node.rs:429-467)
 Entry Octree::pix_get_ref(uint32_t pix) const {
   const uint32_t c = pix_color_index(pix), d = pix_data_index(pix);

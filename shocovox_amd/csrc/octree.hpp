@@ -246,7 +246,17 @@ class Octree {
 
   // Palette management (update/mod.rs:55-136)
   uint32_t add_to_palette(const Entry& e);
-  bool pix_points_to_empty(uint32_t pix) const;
+  // pix_points_to_empty (node.rs:405-427); inline, since brick scans call it
+  // for every voxel
+  bool pix_points_to_empty(uint32_t pix) const {
+    if (pix == EMPTY_MARKER) return true;
+    const uint32_t c = pix_color_index(pix), d = pix_data_index(pix);
+    const bool color_empty =
+        c >= NONE16 || (c < color_palette_.size() ? (color_palette_[c] & 0xFFu) == 0 : true);
+    const bool data_empty =
+        d == NONE16 || (d < data_palette_.size() ? data_palette_[d] == 0 : true);
+    return color_empty && data_empty;
+  }
   Entry pix_get_ref(uint32_t pix) const;
 
   // Read access used by the flattener, the model export and the scene builder.

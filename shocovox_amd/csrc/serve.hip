@@ -461,7 +461,7 @@ int alloc_slot(Slot& sl, uint64_t cap, uint32_t digest_cap, uint32_t batch_jobs)
   sl.digest_cap = digest_cap;
   if (batch_jobs) {
     std::string err;
-    const int st = batch_create(batch_jobs, &sl.batch, &err);
+    const int st = batch_create_variant(batch_jobs, batch_variant_serving(), &sl.batch, &err);
     if (st != SVO_OK) return serve_fail(st, err);
   }
   return SVO_OK;

@@ -502,7 +502,11 @@ hit_found:
 // interleaving across lanes differs, which keeps more lanes active in the
 // brick DDA loop. (Ending a round's first phase early, once a quorum of
 // the wave's lanes wait for their probes, measured slower.)
-template <bool P2, class LutT>
+//
+// CRAWL = false leaves out the crawl_out_of_root shortcut (the restarts are
+// then run one by one, with the same result): it needs three more VGPRs,
+// which builds held to 80 (six waves per SIMD) do not have.
+template <bool P2, class LutT, bool CRAWL = true>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop) {
@@ -534,6 +538,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
   }
   bool finished = target == OOB_OCTANT;
   bool probed = false;  // the current iteration's probe has run
+  bool crawl = false;   // the ray ends in crawl_out_of_root, after the loop
   uint32_t steps = 0;
   for (;;) {
     // phase 1: stack-loop iterations up to the next due probe
@@ -576,9 +581,15 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       uint32_t flat = bitmap_cell(bp);  // cell of the clamped position
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
           (occ & lut[flat * 8u + dir_idx]) == 0ull) {
-        // POP (:445-475)
-        // (no crawl_out_of_root shortcut in this form: it costs 12 VGPRs
-        // here, which spills the 6-waves-per-SIMD builds)
+        // POP (:445-475); a root with nothing ahead of the ray ends the
+        // ray as crawl_out_of_root does, which runs after the loop (where
+        // the loop's state is no longer live)
+        if (CRAWL && stk.count == 1u && key == 0u && !backtrack && target != OOB_OCTANT &&
+            kind != KIND_LEAF) {
+          crawl = true;
+          finished = true;
+          break;
+        }
         stk.pop();
         if (stk.count != 0u) {
           const V3f center = bounds.min + v3u(bounds.size / 2.0f);
@@ -643,6 +654,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       probed = true;
     }
   }
+  if (crawl) r.flags = crawl_out_of_root(t, p, d, steps, r.flags).flags;
   return r;
 }
 

@@ -33,7 +33,8 @@ struct hs_hit {
 
 // mode 0: cast_ray; mode 1: the resumable form (ray_begin, then ray_step
 // until the ray finishes), as the refill kernels run it; mode 2: cast_ray_lod
-// with t->viewing_distance; mode 3: cast_ray_ww. Modes 0, 2 and 3 use the P2 form, as the kernels
+// with t->viewing_distance; mode 3: cast_ray_ww; mode 4: cast_ray_ww without
+// the crawl shortcut. Modes 0, 2 and 3 use the P2 form, as the kernels
 // do, when exact_pow2_sizes holds and `general` is 0, else the general form.
 void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode,
              int general) {
@@ -50,6 +51,9 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int m
     } else if (mode == 3) {
       c = p2 ? svo::cast_ray_ww<true>(v, v.ray_lut, o, d, nullptr, 0u)
              : svo::cast_ray_ww<false>(v, v.ray_lut, o, d, nullptr, 0u);
+    } else if (mode == 4) {
+      c = p2 ? svo::cast_ray_ww<true, uint64_t, false>(v, v.ray_lut, o, d, nullptr, 0u)
+             : svo::cast_ray_ww<false, uint64_t, false>(v, v.ray_lut, o, d, nullptr, 0u);
     } else if (mode == 2) {
       c = p2 ? svo::cast_ray_lod<true>(v, v.ray_lut, o, d, t->viewing_distance)
              : svo::cast_ray_lod<false>(v, v.ray_lut, o, d, t->viewing_distance);

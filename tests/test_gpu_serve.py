@@ -151,3 +151,26 @@ def test_served_deadline_drops_late_requests(gpu_device):
     assert r["timed_out"] > 0 and r["timed_out"] == int((lat == -2).sum())
     assert r["renders"] + r["edits"] + r["timed_out"] + r["rejected"] == r["requests"]
     assert (lat[lat >= 0] >= 0).all() and r["renders"] > 0
+
+
+def test_served_with_a_non_8x8_batch_variant_selected(gpu_device):
+    """The loop's chunked launches are laid out in 8x8 tiles: with a variant of
+    16x4 tiles selected, serving falls back to an 8x8 variant and its results
+    still equal per-request launches."""
+    from shocovox_amd import batch_variants, current_batch_variant, select_batch_variant
+    names = batch_variants()
+    prev = current_batch_variant()
+    cfg = serve.ServeConfig(qps=100000, duration_s=0.003, seed=12, max_inflight=2,
+                            max_batch_rays=8192, max_batch_jobs=64, edit_fraction=0.1,
+                            max_edit_voxels=100, resolutions=SMALL_RES)
+    a, b = _trees(), _trees()
+    _say("serve: burst with classic-w5-g5-16x4 selected")
+    select_batch_variant(names.index("classic-w5-g5-16x4"))
+    try:
+        ra = serve.run(a, [DeviceTree(t) for t in a], cfg, digests=True)
+    finally:
+        select_batch_variant(prev)
+    rb = serve.run(b, [DeviceTree(t) for t in b], dataclasses.replace(cfg, mode="plain"),
+                   digests=True)
+    assert ra["mean_jobs_per_launch"] > 1.5
+    assert np.array_equal(ra["digests"], rb["digests"])

@@ -275,6 +275,7 @@ def test_resumable_traversal_equals_cast_ray():
         assert base.tobytes() == res.tobytes()
         # the while-while arrangement (cast_ray_ww) too
         assert base.tobytes() == hostsim.cast(a, rays, regrouped=True).tobytes()
+        assert base.tobytes() == hostsim.cast(a, rays, regrouped="nocrawl").tobytes()
         exp = oracle.OracleTree(t.export_model()).cast(rays)
         r = compare(exp, {k: res[k] for k in res.dtype.names})
         assert all(v == 0 for k, v in r.items() if k != "rays"), r
@@ -323,7 +324,7 @@ def test_extreme_rays_on_the_host_build():
                  Octree.scene(2, 128, 32, 5)):
         a = tree.flatten().arrays()
         exp = oracle.OracleTree(tree.export_model()).cast(rays)
-        for regrouped in (False, True):
+        for regrouped in (False, True, "nocrawl"):
             got = hostsim.cast(a, rays, regrouped=regrouped)
             res = compare(exp, {k: got[k] for k in got.dtype.names})
             assert all(v == 0 for k, v in res.items() if k != "rays"), (regrouped, res)
@@ -346,14 +347,15 @@ def test_rays_that_leave_the_content_behind():
         o = oracle.OracleTree(tree.export_model())
         for r in (rays, prim):
             exp = o.cast(r)
-            for regrouped in (False, True):
+            for regrouped in (False, True, "nocrawl"):
                 got = hostsim.cast(a, r, regrouped=regrouped)
                 res = compare(exp, {k: got[k] for k in got.dtype.names})
                 assert all(v == 0 for k, v in res.items() if k != "rays"), (size, regrouped, res)
         for limit in (7, 300, 3000):
             g1 = hostsim.cast(a, rays, max_steps=limit)
-            g2 = hostsim.cast(a, rays, max_steps=limit, regrouped=True)
-            assert g1.tobytes() == g2.tobytes(), (size, limit)
+            for form in (True, "nocrawl"):
+                g2 = hostsim.cast(a, rays, max_steps=limit, regrouped=form)
+                assert g1.tobytes() == g2.tobytes(), (size, limit, form)
 
 
 def test_regrouped_traversal_under_forced_guard():
@@ -369,6 +371,8 @@ def test_regrouped_traversal_under_forced_guard():
     for limit in (1, 2, 3, 5, 8, 13):
         base = hostsim.cast(a, rays, max_steps=limit)
         assert base.tobytes() == hostsim.cast(a, rays, max_steps=limit, regrouped=True).tobytes()
+        assert base.tobytes() == hostsim.cast(a, rays, max_steps=limit,
+                                              regrouped="nocrawl").tobytes()
 
 
 def test_forced_iteration_guard_reports_and_ends_rays():
