@@ -519,27 +519,55 @@ int svo_tree_admit(svo_octree* tree, uint64_t budget_bytes, uint64_t reserve_byt
   return SVO_OK;
 }
 
-int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream) {
+}  // extern "C"
+
+namespace svo {
+
+static_assert(sizeof(LayoutMirror::SlotFill) == 12 && sizeof(LayoutMirror::WordWrite) == 12,
+              "patch triples are three packed u32");
+
+int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_event) {
   if (!dtree || !tree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   std::string err;
+  LayoutMirror& m = dtree->mirror;
   if (!dtree->has_mirror) {
     (void)tree->tree.take_dirty();
-    dtree->mirror.build(tree->tree);
+    m.build(tree->tree);
     dtree->has_mirror = true;
   } else {
-    dtree->mirror.apply(tree->tree);
+    m.apply(tree->tree);
   }
   svo_tree_desc d;
-  fill_desc(dtree->mirror.flat(), &d);
+  fill_desc(m.flat(), &d);
   int st = 1;
-  if (!dtree->mirror.full_upload())
-    st = svo::device_tree_patch(dtree->dev, d, dtree->mirror.dirty_records(),
-                                dtree->mirror.dirty_slots(), dtree->mirror.colors_synced(), stream,
-                                &err);
-  if (st == 1) st = svo::device_tree_upload(dtree->dev, d, stream, &err);
+  if (!m.full_upload()) {
+    BrickPatch bp;
+    bp.whole = m.whole_slots().data();
+    bp.n_whole = m.whole_slots().size();
+    bp.fills = reinterpret_cast<const uint32_t*>(m.slot_fills().data());
+    bp.n_fills = m.slot_fills().size();
+    bp.pix_writes = reinterpret_cast<const uint32_t*>(m.pix_writes().data());
+    bp.n_pix = m.pix_writes().size();
+    bp.occ_writes = reinterpret_cast<const uint32_t*>(m.occ_writes().data());
+    bp.n_occ = m.occ_writes().size();
+    st = device_tree_patch(dtree->dev, d, m.dirty_records(), bp, m.colors_synced(), stream,
+                           start_event, &err);
+  }
+  if (st == 1) {
+    st = start_event ? device_record_event(start_event, stream, &err) : SVO_OK;
+    if (st == SVO_OK) st = device_tree_upload(dtree->dev, d, stream, &err);
+  }
   if (st != SVO_OK) return fail(st, err);
-  dtree->mirror.mark_synced();
+  m.mark_synced();
   return SVO_OK;
+}
+
+}  // namespace svo
+
+extern "C" {
+
+int svo_tree_update(svo_tree* dtree, svo_octree* tree, void* stream) {
+  return svo::tree_update(dtree, tree, stream, nullptr);
 }
 
 int svo_tree_update_stats(const svo_tree* dtree, uint32_t* records, uint32_t* slots) {

@@ -278,6 +278,40 @@ __global__ __launch_bounds__(256) void scatter_records_kernel(NodeRec* __restric
   d4[q] = s4[q];
 }
 
+// Fill of patched brick slots: fills[3k] = slot, fills[3k+1] = pix value,
+// fills[3k+2] = occupancy value.
+__global__ __launch_bounds__(256) void fill_slots_kernel(uint32_t* __restrict__ pix,
+                                                         uint32_t* __restrict__ occ,
+                                                         const uint32_t* __restrict__ fills,
+                                                         uint32_t n, uint32_t n3, uint32_t words) {
+  const uint32_t per = n3 + words;
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<uint64_t>(per) * n) return;
+  const uint32_t f = static_cast<uint32_t>(i / per), w = static_cast<uint32_t>(i % per);
+  const uint64_t slot = fills[3 * f];
+  if (w < n3)
+    pix[slot * n3 + w] = fills[3 * f + 1];
+  else
+    occ[slot * words + (w - n3)] = fills[3 * f + 2];
+}
+
+// Word writes into patched brick slots: {slot, word, value} triples, the
+// first n_pix into pix words, the next n_occ into occupancy words.
+__global__ __launch_bounds__(256) void scatter_words_kernel(uint32_t* __restrict__ pix,
+                                                            uint32_t* __restrict__ occ,
+                                                            const uint32_t* __restrict__ writes,
+                                                            uint32_t n_pix, uint32_t n_occ,
+                                                            uint32_t n3, uint32_t words) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pix + n_occ) return;
+  const uint64_t slot = writes[3 * i];
+  const uint32_t word = writes[3 * i + 1], value = writes[3 * i + 2];
+  if (i < n_pix)
+    pix[slot * n3 + word] = value;
+  else
+    occ[slot * words + word] = value;
+}
+
 // Scatter of patched bricks: words [n3 + occ_words] per slot, packed in src.
 __global__ __launch_bounds__(256) void scatter_bricks_kernel(
     uint32_t* __restrict__ pix, uint32_t* __restrict__ occ, const uint32_t* __restrict__ src,
@@ -693,41 +727,57 @@ int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err) {
   return SVO_OK;
 }
 
+int device_record_event(void* event, void* stream, std::string* err) {
+  const hipError_t e =
+      hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? SVO_OK : hip_fail(e, "hipEventRecord", err);
+}
+
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<uint32_t>& records,
-                      const std::vector<uint32_t>& slots, uint32_t colors_from, void* stream,
+                      const BrickPatch& bp, uint32_t colors_from, void* stream, void* start_event,
                       std::string* err) {
   if (d.brick_dim != t->view.dim) return 1;  // caller re-uploads everything
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(t->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
+  const uint32_t n3 = d.brick_dim * d.brick_dim * d.brick_dim;
+  const uint32_t words = occ_words_per_brick(d.brick_dim);
   for (uint32_t r : records)
     if (r >= d.node_count) {
       *err = "patched record out of range";
       return SVO_INVALID_ARGUMENT;
     }
-  for (uint32_t b : slots)
-    if (b >= d.brick_count) {
-      *err = "patched brick slot out of range";
-      return SVO_INVALID_ARGUMENT;
-    }
-  if (d.node_count > t->node_capacity || d.brick_count > t->brick_capacity ||
-      d.color_count > t->color_capacity) {
-    const int st = grow_resident(t, d, s, err);
-    if (st != SVO_OK) return st;
+  auto slots_ok = [&](const uint32_t* v, size_t n, size_t stride, uint32_t word_limit) {
+    for (size_t i = 0; i < n; ++i)
+      if (v[i * stride] >= d.brick_count || (word_limit && v[i * stride + 1] >= word_limit))
+        return false;
+    return true;
+  };
+  if (!slots_ok(bp.whole, bp.n_whole, 1, 0) || !slots_ok(bp.fills, bp.n_fills, 3, 0) ||
+      !slots_ok(bp.pix_writes, bp.n_pix, 3, n3) || !slots_ok(bp.occ_writes, bp.n_occ, 3, words)) {
+    *err = "patched brick slot or word out of range";
+    return SVO_INVALID_ARGUMENT;
   }
-  const uint32_t n3 = d.brick_dim * d.brick_dim * d.brick_dim;
-  const uint32_t words = occ_words_per_brick(d.brick_dim);
+  const uint64_t per_slot = static_cast<uint64_t>(n3) + words;
+  if (bp.n_pix + bp.n_occ > 0x7FFFFFFFull || (bp.n_fills * per_slot + 255) / 256 > 0x7FFFFFFFull ||
+      (bp.n_whole * per_slot + 255) / 256 > 0x7FFFFFFFull) {
+    *err = "patch too large";
+    return SVO_INVALID_ARGUMENT;
+  }
   const uint32_t new_colors = d.color_count > colors_from ? d.color_count - colors_from : 0u;
   // Staging layout (16-byte aligned sections), identical on host and device:
-  // [records][record indices][brick words][slot indices][new colours].
+  // [records][record indices][whole slots' words][whole slot indices]
+  // [fills][pix writes][occupancy writes][new colours].
   // The host side is pinned memory, so the whole patch is one asynchronous
   // DMA on the caller's stream: a copy from pageable memory, or a hipFree,
   // would wait for other streams' work and stall behind busy launches.
   auto align16 = [](size_t v) { return (v + 15) & ~size_t(15); };
   const size_t o_ridx = align16(records.size() * sizeof(NodeRec));
   const size_t o_bw = align16(o_ridx + records.size() * 4);
-  const size_t o_slots = align16(o_bw + slots.size() * static_cast<size_t>(n3 + words) * 4);
-  const size_t o_colors = align16(o_slots + slots.size() * 4);
+  const size_t o_slots = align16(o_bw + bp.n_whole * static_cast<size_t>(n3 + words) * 4);
+  const size_t o_fills = align16(o_slots + bp.n_whole * 4);
+  const size_t o_writes = align16(o_fills + bp.n_fills * 12);
+  const size_t o_colors = align16(o_writes + (bp.n_pix + bp.n_occ) * 12);
   const size_t bytes = align16(o_colors + new_colors * 4u);
   // the set the patch two calls back used: its copy and scatter must have
   // finished before the host refills it
@@ -739,14 +789,27 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   for (size_t i = 0; i < records.size(); ++i)
     std::memcpy(h + i * sizeof(NodeRec), &d.nodes[records[i]], sizeof(NodeRec));
   if (!records.empty()) std::memcpy(h + o_ridx, records.data(), records.size() * 4);
-  for (size_t i = 0; i < slots.size(); ++i) {
+  for (size_t i = 0; i < bp.n_whole; ++i) {
     uint32_t* dst = reinterpret_cast<uint32_t*>(h + o_bw) + i * (n3 + words);
-    std::memcpy(dst, d.brick_pix + static_cast<size_t>(slots[i]) * n3, n3 * sizeof(uint32_t));
-    std::memcpy(dst + n3, d.brick_occ + static_cast<size_t>(slots[i]) * words,
+    std::memcpy(dst, d.brick_pix + static_cast<size_t>(bp.whole[i]) * n3, n3 * sizeof(uint32_t));
+    std::memcpy(dst + n3, d.brick_occ + static_cast<size_t>(bp.whole[i]) * words,
                 words * sizeof(uint32_t));
   }
-  if (!slots.empty()) std::memcpy(h + o_slots, slots.data(), slots.size() * 4);
+  if (bp.n_whole) std::memcpy(h + o_slots, bp.whole, bp.n_whole * 4);
+  if (bp.n_fills) std::memcpy(h + o_fills, bp.fills, bp.n_fills * 12);
+  if (bp.n_pix) std::memcpy(h + o_writes, bp.pix_writes, bp.n_pix * 12);
+  if (bp.n_occ) std::memcpy(h + o_writes + bp.n_pix * 12, bp.occ_writes, bp.n_occ * 12);
   if (new_colors) std::memcpy(h + o_colors, d.colors + colors_from, new_colors * 4u);
+  // host work done: the device part starts here
+  if (start_event) {
+    e = hipEventRecord(static_cast<hipEvent_t>(start_event), s);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord(patch start)", err);
+  }
+  if (d.node_count > t->node_capacity || d.brick_count > t->brick_capacity ||
+      d.color_count > t->color_capacity) {
+    const int st = grow_resident(t, d, s, err);
+    if (st != SVO_OK) return st;
+  }
   char* base = static_cast<char*>(ps.dev);
   if (bytes > 0) {
     e = hipMemcpyAsync(base, h, bytes, hipMemcpyHostToDevice, s);
@@ -760,15 +823,34 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "scatter_records_kernel", err);
   }
-  if (!slots.empty()) {
-    const uint64_t total = static_cast<uint64_t>(slots.size()) * (n3 + words);
+  if (bp.n_fills) {
+    const uint64_t total = static_cast<uint64_t>(bp.n_fills) * (n3 + words);
+    hipLaunchKernelGGL(fill_slots_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)),
+                       dim3(256), 0, s, t->brick_pix, t->brick_occ,
+                       reinterpret_cast<const uint32_t*>(base + o_fills),
+                       static_cast<uint32_t>(bp.n_fills), n3, words);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "fill_slots_kernel", err);
+  }
+  if (bp.n_whole) {
+    const uint64_t total = static_cast<uint64_t>(bp.n_whole) * (n3 + words);
     hipLaunchKernelGGL(scatter_bricks_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)),
                        dim3(256), 0, s, t->brick_pix, t->brick_occ,
                        reinterpret_cast<const uint32_t*>(base + o_bw),
                        reinterpret_cast<const uint32_t*>(base + o_slots),
-                       static_cast<uint32_t>(slots.size()), n3, words);
+                       static_cast<uint32_t>(bp.n_whole), n3, words);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "scatter_bricks_kernel", err);
+  }
+  if (bp.n_pix + bp.n_occ) {
+    const uint32_t n = static_cast<uint32_t>(bp.n_pix + bp.n_occ);
+    hipLaunchKernelGGL(scatter_words_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                       t->brick_pix, t->brick_occ,
+                       reinterpret_cast<const uint32_t*>(base + o_writes),
+                       static_cast<uint32_t>(bp.n_pix), static_cast<uint32_t>(bp.n_occ), n3,
+                       words);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "scatter_words_kernel", err);
   }
   if (new_colors) {
     e = hipMemcpyAsync(t->colors + colors_from, base + o_colors, new_colors * 4u,

@@ -63,17 +63,37 @@ uint64_t device_tree_bytes_for(const svo_tree_desc& desc);
 // Makes both patch stagings (pinned host and device) hold at least `bytes`
 // (at least 8 MiB), so the first edits of a tree allocate nothing.
 int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err);
-// Writes the given records and brick slots of a layout into an existing tree,
-// asynchronously on `stream`: the call returns once the copy and scatter are
-// queued (it waits on the host only for the patch two calls back, whose
-// staging it reuses). Arrays whose capacity `desc` (the layout's full
-// description) outgrows are reallocated larger and their live contents
+// The brick part of an incremental patch: slots sent whole (their words are
+// read from the layout), slot fills {slot, pix value, occupancy value}, and
+// word writes {slot, word, value} into the slots' pix and occupancy words,
+// applied after the fills. Each listed slot appears in one form only.
+struct BrickPatch {
+  const uint32_t* whole = nullptr;
+  size_t n_whole = 0;
+  const uint32_t* fills = nullptr;  // 3 words per fill
+  size_t n_fills = 0;
+  const uint32_t* pix_writes = nullptr;  // 3 words per write
+  size_t n_pix = 0;
+  const uint32_t* occ_writes = nullptr;  // 3 words per write
+  size_t n_occ = 0;
+};
+
+// Writes the given records and brick changes of a layout into an existing
+// tree, asynchronously on `stream`: the call returns once the copy and
+// scatter are queued (it waits on the host only for the patch two calls
+// back, whose staging it reuses). Arrays whose capacity `desc` (the layout's
+// full description) outgrows are reallocated larger and their live contents
 // copied device to device on `stream` first, so a patch never re-sends the
-// whole tree from the host. Returns 1 (asking for a full upload) only when
-// the brick dimension differs.
+// whole tree from the host. `start_event` (a hipEvent_t, or NULL) is
+// recorded on `stream` before the patch's first device operation, after its
+// host work. Returns 1 (asking for a full upload) only when the brick
+// dimension differs.
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& desc,
-                      const std::vector<uint32_t>& records, const std::vector<uint32_t>& slots,
-                      uint32_t colors_from, void* stream, std::string* err);
+                      const std::vector<uint32_t>& records, const BrickPatch& bricks,
+                      uint32_t colors_from, void* stream, void* start_event, std::string* err);
+
+// hipEventRecord(event, stream) with the status mapped.
+int device_record_event(void* event, void* stream, std::string* err);
 
 int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hits& out,
                 void* stream, std::string* err);

@@ -42,7 +42,7 @@ void parallel_range(size_t n, F&& f) {
   for (auto& th : pool) th.join();
 }
 
-FlatTree flatten(const Octree& tree, bool headroom) {
+FlatTree flatten(const Octree& tree, bool headroom, std::vector<uint64_t>* versions) {
   FlatTree f;
   f.octree_size = tree.size();
   f.brick_dim = tree.brick_dim();
@@ -80,10 +80,10 @@ FlatTree flatten(const Octree& tree, bool headroom) {
 
   // Records, with Parted bricks numbered in record order; the brick words
   // are then copied and their occupancy sidecars built in parallel.
-  auto with_room = [headroom](size_t n) { return headroom ? n + n / 4 + 64 : n; };
+  auto with_room = [headroom](size_t n) { return headroom ? 2 * n + 64 : n; };
   f.nodes.reserve(with_room(order.size()));
   f.nodes.resize(order.size());
-  std::vector<const std::vector<uint32_t>*> parted;
+  std::vector<const Brick*> parted;
   auto encode_brick = [&](const Brick& b, uint32_t octant, NodeRec& r) {
     uint32_t form = FORM_EMPTY;
     uint32_t word = 0;
@@ -93,7 +93,7 @@ FlatTree flatten(const Octree& tree, bool headroom) {
     } else if (b.kind == BrickKind::Parted) {
       form = FORM_PARTED;
       word = static_cast<uint32_t>(parted.size());
-      parted.push_back(&b.parted);
+      parted.push_back(&b);
     }
     r.meta |= form << (8u + 2u * octant);
     r.child[octant] = word;
@@ -113,7 +113,7 @@ FlatTree flatten(const Octree& tree, bool headroom) {
     } else if (mip.kind == BrickKind::Parted) {
       r.mip = FORM_PARTED;
       r.mip_word = static_cast<uint32_t>(parted.size());
-      parted.push_back(&mip.parted);
+      parted.push_back(&mip);
     }
     switch (n.kind) {
       case NodeKind::Nothing:
@@ -141,13 +141,17 @@ FlatTree flatten(const Octree& tree, bool headroom) {
   f.brick_pix.resize(parted.size() * n3);
   f.brick_occ.assign(parted.size() * words, 0u);
   parallel_range(parted.size(), [&](size_t slot) {
-    const std::vector<uint32_t>& voxels = *parted[slot];
+    const std::vector<uint32_t>& voxels = parted[slot]->parted;
     uint32_t* pix = f.brick_pix.data() + slot * n3;
     uint32_t* occ = f.brick_occ.data() + slot * words;
     std::memcpy(pix, voxels.data(), n3 * sizeof(uint32_t));
     for (size_t i = 0; i < n3; ++i)
       if (!tree.pix_points_to_empty(voxels[i])) occ[i / 32] |= 1u << (i % 32);
   });
+  if (versions) {
+    versions->resize(parted.size());
+    for (size_t i = 0; i < parted.size(); ++i) (*versions)[i] = parted[i]->version;
+  }
   return f;
 }
 
@@ -236,8 +240,9 @@ constexpr uint32_t NO_RECORD = 0xFFFFFFFFu;
 }
 
 void LayoutMirror::build(const Octree& tree) {
-  flat_ = flatten(tree, true);
-  key_to_rec_.reserve(tree.nodes().len() + tree.nodes().len() / 4 + 64);
+  flat_ = flatten(tree, true, &slot_version_);
+  slot_version_.reserve(2 * slot_version_.size() + 64);
+  key_to_rec_.reserve(2 * tree.nodes().len() + 64);
   key_to_rec_.assign(tree.nodes().len(), NO_RECORD);
   // recover key -> record from the breadth-first order flatten() used
   std::vector<uint32_t> order{Octree::ROOT_NODE_KEY};
@@ -261,8 +266,14 @@ void LayoutMirror::build(const Octree& tree) {
   const size_t n3 = static_cast<size_t>(flat_.brick_dim) * flat_.brick_dim * flat_.brick_dim;
   slot_dirty_flag_.reserve(n3 ? flat_.brick_pix.capacity() / n3 : 0);
   slot_dirty_flag_.assign(flat_.brick_count(), 0);
+  whole_slots_.clear();
+  fills_.clear();
+  pix_writes_.clear();
+  occ_writes_.clear();
+  rewritten_ = false;
   full_ = true;
   colors_synced_ = 0;
+  synced_slots_ = 0;
 }
 
 uint32_t LayoutMirror::record_for(uint32_t key) {
@@ -308,9 +319,7 @@ uint32_t LayoutMirror::write_brick(const Octree& tree, const Brick& b, uint32_t 
   const size_t n3 = static_cast<size_t>(dim) * dim * dim;
   const uint32_t words = occ_words_per_brick(dim);
   uint32_t slot = old_slot;
-  bool fresh = false;
   if (slot == NO_RECORD) {
-    fresh = true;
     if (!free_slots_.empty()) {
       slot = free_slots_.back();
       free_slots_.pop_back();
@@ -319,20 +328,121 @@ uint32_t LayoutMirror::write_brick(const Octree& tree, const Brick& b, uint32_t 
       flat_.brick_pix.resize(flat_.brick_pix.size() + n3, EMPTY_MARKER);
       flat_.brick_occ.resize(flat_.brick_occ.size() + words, 0u);
       slot_dirty_flag_.push_back(0);
+      slot_version_.push_back(0);
     }
+  } else if (b.version != 0 && slot_version_[slot] == b.version) {
+    return slot;  // the same content as the slot's last write
   }
-  uint32_t* pix = flat_.brick_pix.data() + static_cast<size_t>(slot) * n3;
-  if (!fresh && std::equal(b.parted.begin(), b.parted.end(), pix)) return slot;
-  std::copy(b.parted.begin(), b.parted.end(), pix);
-  uint32_t* occ = flat_.brick_occ.data() + static_cast<size_t>(slot) * words;
-  std::fill(occ, occ + words, 0u);
+  slot_version_[slot] = b.version;
+  occ_scratch_.assign(words, 0u);
   for (size_t i = 0; i < n3; ++i)
-    if (!tree.pix_points_to_empty(pix[i])) occ[i / 32] |= 1u << (i % 32);
-  if (!slot_dirty_flag_[slot]) {
+    if (!tree.pix_points_to_empty(b.parted[i])) occ_scratch_[i / 32] |= 1u << (i % 32);
+  encode_slot(slot, b.parted.data(), occ_scratch_.data());
+  return slot;
+}
+
+// Puts a slot's new contents into the mirror and lists the cheapest form that
+// brings the device's copy of the slot to them.
+void LayoutMirror::encode_slot(uint32_t slot, const uint32_t* new_pix, const uint32_t* new_occ) {
+  const uint32_t dim = flat_.brick_dim;
+  const uint32_t n3 = dim * dim * dim;
+  const uint32_t words = occ_words_per_brick(dim);
+  uint32_t* pix = flat_.brick_pix.data() + static_cast<size_t>(slot) * n3;
+  uint32_t* occ = flat_.brick_occ.data() + static_cast<size_t>(slot) * words;
+  const bool again = slot_dirty_flag_[slot] != 0;
+  // the device holds the mirror's words for a slot it has received and that
+  // this patch has not written yet
+  const bool known = slot < synced_slots_ && !again;
+  // one pass: words that differ from the device's copy, and from the two
+  // candidate fill values (the first and the last word)
+  const uint32_t p0 = new_pix[0], p1 = new_pix[n3 - 1];
+  const uint32_t o0 = new_occ[0], o1 = new_occ[words - 1];
+  size_t dk = 0, dp0 = 0, dp1 = 0, do0 = 0, do1 = 0;
+  for (uint32_t i = 0; i < n3; ++i) {
+    const uint32_t a = new_pix[i];
+    dk += a != pix[i];
+    dp0 += a != p0;
+    dp1 += a != p1;
+  }
+  for (uint32_t i = 0; i < words; ++i) {
+    const uint32_t a = new_occ[i];
+    dk += a != occ[i];
+    do0 += a != o0;
+    do1 += a != o1;
+  }
+  if (known && dk == 0) return;
+  if (!again) {
     slot_dirty_flag_[slot] = 1;
     dirty_slots_.push_back(slot);
   }
-  return slot;
+  const uint32_t fp = dp0 <= dp1 ? p0 : p1, fo = do0 <= do1 ? o0 : o1;
+  const size_t d_fill = std::min(dp0, dp1) + std::min(do0, do1);
+  const size_t w = sizeof(WordWrite);
+  const size_t whole_cost = (static_cast<size_t>(n3) + words + 1) * sizeof(uint32_t);
+  const size_t fill_cost = sizeof(SlotFill) + d_fill * w;
+  const size_t known_cost = known ? dk * w : SIZE_MAX;
+  if (again) {
+    // its earlier form in this patch is superseded: send it whole
+    whole_slots_.push_back(slot);
+    rewritten_ = true;
+  } else if (whole_cost <= fill_cost && whole_cost <= known_cost) {
+    whole_slots_.push_back(slot);
+  } else {
+    const bool fill = fill_cost < known_cost;
+    if (fill) {
+      fills_.push_back(SlotFill{slot, fp, fo});
+      for (uint32_t i = 0; i < n3; ++i)
+        if (new_pix[i] != fp) pix_writes_.push_back(WordWrite{slot, i, new_pix[i]});
+      for (uint32_t i = 0; i < words; ++i)
+        if (new_occ[i] != fo) occ_writes_.push_back(WordWrite{slot, i, new_occ[i]});
+    } else {
+      for (uint32_t i = 0; i < n3; ++i)
+        if (new_pix[i] != pix[i]) pix_writes_.push_back(WordWrite{slot, i, new_pix[i]});
+      for (uint32_t i = 0; i < words; ++i)
+        if (new_occ[i] != occ[i]) occ_writes_.push_back(WordWrite{slot, i, new_occ[i]});
+    }
+  }
+  std::copy(new_pix, new_pix + n3, pix);
+  std::copy(new_occ, new_occ + words, occ);
+}
+
+// After a slot was written twice in one patch: it is sent whole, so its
+// fills and word writes are dropped (and duplicate whole entries merged).
+void LayoutMirror::drop_partial_forms() {
+  std::vector<uint8_t> whole(flat_.brick_count(), 0);
+  std::vector<uint32_t> slots;
+  for (uint32_t s : whole_slots_)
+    if (!whole[s]) {
+      whole[s] = 1;
+      slots.push_back(s);
+    }
+  whole_slots_.swap(slots);
+  auto keep_fill = [&](const SlotFill& f) { return whole[f.slot] != 0; };
+  auto keep_word = [&](const WordWrite& w) { return whole[w.slot] != 0; };
+  fills_.erase(std::remove_if(fills_.begin(), fills_.end(), keep_fill), fills_.end());
+  pix_writes_.erase(std::remove_if(pix_writes_.begin(), pix_writes_.end(), keep_word),
+                    pix_writes_.end());
+  occ_writes_.erase(std::remove_if(occ_writes_.begin(), occ_writes_.end(), keep_word),
+                    occ_writes_.end());
+  rewritten_ = false;
+}
+
+void LayoutMirror::mark_synced() {
+  last_recs_ = full_ ? static_cast<uint32_t>(flat_.nodes.size())
+                     : static_cast<uint32_t>(dirty_recs_.size());
+  last_slots_ = full_ ? flat_.brick_count() : static_cast<uint32_t>(dirty_slots_.size());
+  for (uint32_t r : dirty_recs_) rec_dirty_flag_[r] = 0;
+  for (uint32_t s : dirty_slots_) slot_dirty_flag_[s] = 0;
+  dirty_recs_.clear();
+  dirty_slots_.clear();
+  whole_slots_.clear();
+  fills_.clear();
+  pix_writes_.clear();
+  occ_writes_.clear();
+  rewritten_ = false;
+  full_ = false;
+  colors_synced_ = static_cast<uint32_t>(flat_.colors.size());
+  synced_slots_ = flat_.brick_count();
 }
 
 void LayoutMirror::write_record(const Octree& tree, uint32_t key, std::vector<uint32_t>* pending) {
@@ -423,8 +533,7 @@ void LayoutMirror::apply(Octree& tree) {
     done[k] = 1;
     write_record(tree, k, &pending);
   }
-  for (uint32_t r : dirty_recs_) rec_dirty_flag_[r] = 0;
-  for (uint32_t s : dirty_slots_) slot_dirty_flag_[s] = 0;
+  if (rewritten_) drop_partial_forms();
 }
 
 }  // namespace svo

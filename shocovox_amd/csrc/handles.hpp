@@ -30,4 +30,8 @@ struct svo_tree {
 namespace svo {
 // Sets the message svo_last_error() returns on the calling thread.
 void set_last_error(const std::string& msg);
+// svo_tree_update that also records `start_event` (a hipEvent_t, or NULL) on
+// `stream` when the host part of the update is done, before its first device
+// operation.
+int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_event);
 }  // namespace svo

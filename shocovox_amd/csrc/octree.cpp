@@ -458,6 +458,27 @@ bool Octree::brick_is_part_empty_throughout(const Brick& b, uint32_t part_octant
 uint64_t Octree::brick_occupied_bits(const std::vector<uint32_t>& brick) const {
   const uint64_t dim = brick_dim_;
   uint64_t bitmap = 0;
+  if (dim >= 4 && (dim & (dim - 1)) == 0) {
+    // For these dimensions set_occupancy_in_bitmap_64bits(p, 1, dim) sets the
+    // single bit of cell p / (dim / 4): the bitmap is one bit per block of
+    // (dim/4)^3 voxels, set when any voxel of the block is non-empty.
+    const uint64_t b = dim / 4;
+    for (uint64_t z = 0; z < dim; ++z)
+      for (uint64_t y = 0; y < dim; ++y) {
+        const uint32_t* row = brick.data() + flat_projection(0, y, z, dim);
+        const uint64_t base = 4 * (y / b) + 16 * (z / b);
+        for (uint64_t bx = 0; bx < 4; ++bx) {
+          const uint64_t bit = 1ull << (base + bx);
+          if (bitmap & bit) continue;
+          for (uint64_t i = 0; i < b; ++i)
+            if (!pix_points_to_empty(row[bx * b + i])) {
+              bitmap |= bit;
+              break;
+            }
+        }
+      }
+    return bitmap;
+  }
   for (uint64_t x = 0; x < dim; ++x)
     for (uint64_t y = 0; y < dim; ++y)
       for (uint64_t z = 0; z < dim; ++z)
@@ -778,6 +799,7 @@ uint64_t Octree::leaf_update(bool overwrite_if_empty, uint32_t node_key, Cube no
         }
         return 0;
       }
+      b.touch();
       return update_brick(overwrite_if_empty, b.parted, target_bounds, dim, pos, size, tc);
     }
     case NodeKind::UniformLeaf: {
@@ -815,8 +837,10 @@ uint64_t Octree::leaf_update(bool overwrite_if_empty, uint32_t node_key, Cube no
         if (1 < dim && ((pix_points_to_empty(tc) && pix_points_to_empty(brick[f])) ||
                         (!pix_points_to_empty(tc) && brick[f] == tc)))
           return 0;
-        if (node_bounds.size <= static_cast<float>(dim) && dim > 1)
+        if (node_bounds.size <= static_cast<float>(dim) && dim > 1) {
+          mat.touch();
           return update_brick(overwrite_if_empty, brick, node_bounds, dim, pos, size, tc);
+        }
         NodeContent leaf;
         leaf.kind = NodeKind::Leaf;
         uint64_t us = 0;
@@ -1585,6 +1609,7 @@ void Octree::update_mip(uint32_t key, Cube bounds, const uint32_t pos[3]) {
     m = Brick::make_parted(std::vector<uint32_t>(static_cast<size_t>(dim) * dim * dim, m.solid));
   }
   m.parted[f] = entry;
+  m.touch();
 }
 
 Entry Octree::sample_root_mip(uint32_t octant, uint32_t x, uint32_t y, uint32_t z) const {

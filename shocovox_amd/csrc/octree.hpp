@@ -13,6 +13,7 @@
 //            src/octree/update/{mod,insert,clear}.rs
 #pragma once
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -25,10 +26,21 @@ namespace svo {
 
 enum class BrickKind : uint8_t { Empty = 0, Solid = 1, Parted = 2 };
 
+// A fresh, process-wide unique content version for a Parted brick (never 0).
+inline uint64_t next_brick_version() {
+  static std::atomic<uint64_t> counter{0};
+  return counter.fetch_add(1, std::memory_order_relaxed) + 1;
+}
+
 struct Brick {
   BrickKind kind = BrickKind::Empty;
   uint32_t solid = 0;
   std::vector<uint32_t> parted;
+  // Content version of a Parted brick: two bricks with the same non-zero
+  // version hold the same values (a copy keeps it; every write in place
+  // takes a fresh one through touch()). 0 = unknown. The layout mirror uses
+  // it to skip bricks an edit did not change; equality ignores it.
+  uint64_t version = 0;
 
   static Brick empty() { return Brick{}; }
   static Brick make_solid(uint32_t v) {
@@ -41,8 +53,11 @@ struct Brick {
     Brick b;
     b.kind = BrickKind::Parted;
     b.parted = std::move(v);
+    b.version = next_brick_version();
     return b;
   }
+  // Call after writing `parted` in place.
+  void touch() { version = next_brick_version(); }
   bool operator==(const Brick& o) const {
     if (kind != o.kind) return false;
     if (kind == BrickKind::Solid) return solid == o.solid;

@@ -398,13 +398,13 @@ void edit_worker_main(Session* s) {
       if (e == hipSuccess && job.after_edit)
         e = hipStreamWaitEvent(stream, s->tree_ready[q.tree], 0);
       st = apply_edit(s->trees[q.tree]->tree, q);
-      et.applied = s->now();
-      if (e == hipSuccess) e = hipEventRecord(job.start, stream);
       if (e != hipSuccess) {
         st = SVO_HIP_ERROR;
         err = std::string("edit ordering: ") + hipGetErrorString(e);
       } else {
-        if (st == SVO_OK) st = svo_tree_update(s->dtrees[q.tree], s->trees[q.tree], stream);
+        // the layout update and the patch's staging are host work; job.start
+        // marks the start of the patch's device work
+        if (st == SVO_OK) st = tree_update(s->dtrees[q.tree], s->trees[q.tree], stream, job.start);
         if (st == SVO_OK) (void)svo_tree_update_stats(s->dtrees[q.tree], &recs, &bricks);
         err = st == SVO_OK ? "" : svo_last_error();
         if (st == SVO_OK && (hipEventRecord(job.done, stream) != hipSuccess ||
@@ -413,6 +413,7 @@ void edit_worker_main(Session* s) {
           err = "edit event record failed";
         }
       }
+      et.applied = s->now();
     }
     std::lock_guard<std::mutex> g(w.mu);
     if (st != SVO_OK && w.status == SVO_OK) {

@@ -84,3 +84,35 @@ def result_digest(h) -> int:
     for f in (h["hit"], h["albedo"], vox[:, 0], vox[:, 1], vox[:, 2], dist):
         a = _mix64(a ^ np.asarray(f).astype(np.uint32).astype(np.uint64))
     return int(a.sum(dtype=np.uint64))
+
+
+def canonical_layout(a: dict):
+    """The device layout `a` (FlatTree.arrays()) as a structure independent of
+    record and slot numbering: from the root record, each record's kind,
+    occupancy, MIP brick and children (Internal) or bricks per octant (Leaf,
+    UniformLeaf), where a Parted brick is represented by its pix words and
+    its occupancy-sidecar words. Two layouts of the same tree must give equal
+    structures, whatever order their records and slots were allocated in."""
+    nodes, pix, occ = a["nodes"], a["brick_pix"], a["brick_occ"]
+    n3 = a["brick_dim"] ** 3
+    words = (n3 + 31) // 32
+
+    def brick(form, word):
+        if form == 2:
+            s = int(word)
+            return (2, pix[s * n3:(s + 1) * n3].tobytes(), occ[s * words:(s + 1) * words].tobytes())
+        return (int(form), int(word) if form == 1 else 0)
+
+    def rec(i):
+        r = nodes[i]
+        kind = int(r["meta"]) & 3
+        out = [kind, int(r["occ"]), brick(int(r["mip"]), r["mip_word"])]
+        for o in range(8):
+            c = int(r["child"][o])
+            if kind == 1:
+                out.append(None if c == 0xFFFFFFFF else rec(c))
+            elif kind in (2, 3):
+                out.append(brick((int(r["meta"]) >> (8 + 2 * o)) & 3, c))
+        return tuple(out)
+
+    return rec(0)

@@ -19,7 +19,7 @@ import pytest
 import hostsim
 import oracle
 from kat_trees import fixed_kats
-from parity import compare
+from parity import canonical_layout, compare
 from shocovox_amd import Octree, OctreeEntry, _native, workloads as W
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -234,6 +234,8 @@ def test_incremental_layout_matches_fresh_tree(seed):
                 tree.clear_at_lod(p, s)
         flat.sync(tree)
         a = flat.arrays()
+        # word for word what a fresh flatten of the edited tree holds
+        assert canonical_layout(a) == canonical_layout(tree.flatten().arrays())
         exp = oracle.OracleTree(tree.export_model()).cast(rays)
         got = hostsim.cast(a, rays)
         res = compare(exp, {k: got[k] for k in got.dtype.names})

@@ -9,7 +9,7 @@ import pytest
 
 import hostsim
 import oracle
-from parity import assert_parity as _assert_parity
+from parity import assert_parity as _assert_parity, canonical_layout
 from shocovox_amd import MIPResamplingMethods as M, Octree, OctreeEntry, workloads as W
 
 F32_MAX = float(np.finfo(np.float32).max)
@@ -137,5 +137,6 @@ def test_edited_mips_reach_the_mirror():
         else:
             t.insert_many(xyz, ents)
         flat.sync(t)
+        assert canonical_layout(flat.arrays()) == canonical_layout(t.flatten().arrays())
         got = hostsim.cast(flat.arrays(), rays, viewing_distance=3.0)
         assert_parity(_oracle(t).cast(rays, viewing_distance=3.0), got, f"round {rnd}")
