@@ -507,8 +507,8 @@ def spheres_block(args) -> dict:
     launch plus a cast launch."""
     import torch
 
-    from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variants,
-                              select_batch_variant, workloads as W)
+    from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variant_scratch,
+                              batch_variants, select_batch_variant, workloads as W)
 
     cfg = W.SPHERES
     t0 = time.time()
@@ -550,7 +550,8 @@ def spheres_block(args) -> dict:
         jobs = [Batch.render_job(d, vp, cfg.width, cfg.height, pr), Batch.cast_job(d, inc, pc)]
         el = timed(lambda: bt.run(jobs, stream))
         same = all(torch.equal(b[k], r[k]) for b, r in zip((pr, pc), ref) for k in fields)
-        tuning[name] = {"ms": round(el * 1e3, 3), "equal_to_launches": bool(same)}
+        tuning[name] = {"ms": round(el * 1e3, 3), "equal_to_launches": bool(same),
+                        "scratch_bytes_per_lane": batch_variant_scratch(names.index(name))}
         log(f"spheres batched {name}: {el * 1e3:.3f} ms, equal {same}")
         del bt
     best = min((n for n in tuning if tuning[n]["equal_to_launches"]),
@@ -636,8 +637,9 @@ def main() -> int:
     if args.mode == "lod":
         return lod_mode(args)
 
-    from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variants, collect_kernel_ms,
-                              current_batch_variant, render_variants, select_batch_variant,
+    from shocovox_amd import (Batch, DeviceTree, alloc_hits, batch_variant_scratch,
+                              batch_variants, collect_kernel_ms, current_batch_variant,
+                              render_variant_scratch, render_variants, select_batch_variant,
                               select_render_variant, set_timing, workloads as W)
 
     cfg = W.CONFIGS[args.workload]
@@ -669,11 +671,15 @@ def main() -> int:
             dtree.cast_into(incoherent[0], incoherent[0].shape[0], incoherent[1], stream)
 
     # one launch per pose: autotune the render kernel's launch variants
+    # (builds that spill registers to scratch are not candidates)
     names = render_variants()
+    render_scratch = {n: render_variant_scratch(i) for i, n in enumerate(names)}
     tuning = {}
     if args.variant == "auto":
         set_timing(True)
         for i, name in enumerate(names):
+            if render_scratch[name]:
+                continue
             select_render_variant(i)
             per_frame_step()
             collect_kernel_ms()
@@ -721,6 +727,7 @@ def main() -> int:
     batch = None
     batch_tuning = {}
     bnames = batch_variants()
+    batch_scratch = {n: batch_variant_scratch(i) for i, n in enumerate(bnames)}
     if args.launch == "batched":
         outs = bufs + ([incoherent[1]] if incoherent is not None else [])
 
@@ -737,6 +744,8 @@ def main() -> int:
             for vi, name in enumerate(bnames):
                 if name.startswith("r1-"):
                     continue  # lane refill: measured 2x slower on coherent frames
+                if batch_scratch[name]:
+                    continue  # spills registers to scratch
                 bt, jobs = make(vi)
                 bt.run(jobs, stream)
                 _, ms = time_steps(lambda: bt.run(jobs, stream), 3)
@@ -851,6 +860,9 @@ def main() -> int:
                    "batch_variant": bvariant, "batch_autotune_ms": batch_tuning or None,
                    "outputs_equal_to_per_frame": equal, "per_frame": per_frame,
                    "launch_variant": best, "autotune_ms": tuning or None,
+                   "scratch_bytes_per_lane": {
+                       "batch": {n: b for n, b in batch_scratch.items() if b},
+                       "render": {n: b for n, b in render_scratch.items() if b}},
                    "tree": W.tree_statistics(tree)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -408,6 +408,9 @@ int svo_batch_variant_count(void);
 const char* svo_batch_variant_name(int variant);
 int svo_select_batch_variant(int variant);
 int svo_current_batch_variant(void);
+/* Private (scratch) bytes per lane of a variant's kernel build; non-zero
+ * means the build spills registers (needs the HIP runtime). */
+int svo_batch_variant_scratch(int variant, uint32_t* bytes);
 
 /* ---------------------------------------------------------------------- */
 /* Serving loop (no counterpart in the reference): ray-cast requests arrive */
@@ -535,6 +538,9 @@ int svo_render_variant_count(void);
 const char* svo_render_variant_name(int variant);
 int svo_select_render_variant(int variant);
 int svo_current_render_variant(void);
+/* Private (scratch) bytes per lane of a variant's kernel build; non-zero
+ * means the build spills registers (needs the HIP runtime). */
+int svo_render_variant_scratch(int variant, uint32_t* bytes);
 
 /* The RAY_TO_NODE_OCCUPANCY_BITMASK_LUT the kernels use, [pos*8 + dir]
  * (src/spatial/lut.rs:255-896), regenerated from the reference's generator

@@ -167,6 +167,7 @@ _SIGS = {
     "svo_batch_variant_name": (C.c_char_p, [C.c_int]),
     "svo_select_batch_variant": (C.c_int, [C.c_int]),
     "svo_current_batch_variant": (C.c_int, []),
+    "svo_batch_variant_scratch": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
     "svo_serve_plan": (C.c_int, [C.POINTER(ServeConfig), _P, _U32, _P, _U64,
                                  C.POINTER(_U64)]),
     "svo_serve_apply_edit": (C.c_int, [_P, C.POINTER(ServeRequest)]),
@@ -178,6 +179,7 @@ _SIGS = {
     "svo_render_variant_name": (C.c_char_p, [C.c_int]),
     "svo_select_render_variant": (C.c_int, [C.c_int]),
     "svo_current_render_variant": (C.c_int, []),
+    "svo_render_variant_scratch": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
     "svo_set_timing": (C.c_int, [C.c_int]),
     "svo_collect_kernel_ms": (_U32, [C.POINTER(C.c_float), _U32]),
 }

@@ -45,6 +45,8 @@ uint64_t batch_tiles(const BatchContext* b);
 // when it was created.
 int batch_variant_count();
 const char* batch_variant_name(int i);
+// Private (scratch) bytes per lane of a variant's kernel (non-zero: spills).
+int batch_variant_scratch(int i, uint32_t* bytes, std::string* err);
 bool batch_variant_select(int i);
 int batch_variant_current();
 int batch_context_variant(const BatchContext* b);

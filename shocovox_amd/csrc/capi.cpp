@@ -717,6 +717,12 @@ int svo_select_batch_variant(int variant) {
   return SVO_OK;
 }
 int svo_current_batch_variant(void) { return svo::batch_variant_current(); }
+int svo_batch_variant_scratch(int variant, uint32_t* bytes) {
+  if (!bytes) return fail(SVO_INVALID_ARGUMENT, "bytes is NULL");
+  std::string err;
+  const int st = svo::batch_variant_scratch(variant, bytes, &err);
+  return st == SVO_OK ? SVO_OK : fail(st, err);
+}
 
 int svo_ray_occupancy_lut(uint64_t* out512) {
   if (!out512) return fail(SVO_INVALID_ARGUMENT, "out is NULL");
@@ -727,6 +733,13 @@ int svo_ray_occupancy_lut(uint64_t* out512) {
 int svo_render_variant_count(void) { return svo::render_variant_count(); }
 
 const char* svo_render_variant_name(int variant) { return svo::render_variant_name(variant); }
+
+int svo_render_variant_scratch(int variant, uint32_t* bytes) {
+  if (!bytes) return fail(SVO_INVALID_ARGUMENT, "bytes is NULL");
+  std::string err;
+  const int st = svo::render_variant_scratch(variant, bytes, &err);
+  return st == SVO_OK ? SVO_OK : fail(st, err);
+}
 
 int svo_select_render_variant(int variant) {
   if (!svo::render_variant_select(variant)) return fail(SVO_INVALID_ARGUMENT, "no such variant");

@@ -422,6 +422,18 @@ bool render_variant_select(int i) {
   return true;
 }
 int render_variant_current() { return g_render_variant; }
+int render_variant_scratch(int i, uint32_t* bytes, std::string* err) {
+  if (i < 0 || i >= kRenderVariantCount) {
+    *err = "no such variant";
+    return SVO_INVALID_ARGUMENT;
+  }
+  hipFuncAttributes a{};
+  const hipError_t e =
+      hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kRenderVariants[i].fn));
+  if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes", err);
+  *bytes = static_cast<uint32_t>(a.localSizeBytes);
+  return SVO_OK;
+}
 
 uint32_t timing_collect(float* out_ms, uint32_t cap) {
   std::lock_guard<std::mutex> g(g_timing_mu);

@@ -114,6 +114,9 @@ int device_render_rgba8(const DeviceTree& t, V3f origin, V3f dir, float glass_w,
 // shape); the process-wide selection applies to later svo_render calls.
 int render_variant_count();
 const char* render_variant_name(int i);
+// Private (scratch) bytes per lane of a variant's kernel: non-zero when the
+// build spills registers.
+int render_variant_scratch(int i, uint32_t* bytes, std::string* err);
 bool render_variant_select(int i);
 int render_variant_current();
 

@@ -55,6 +55,16 @@ SVO_HD inline float rsignum(float x) {
 }
 
 // `f as i32` (saturating, NaN -> 0)
+// Bit (b mod 32) of w.
+SVO_HD inline uint32_t bit_of(uint32_t w, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // v_bfe_u32 reads the low 5 bits of the offset: one instruction
+  return __builtin_amdgcn_ubfe(w, b, 1u);
+#else
+  return (w >> (b & 31u)) & 1u;
+#endif
+}
+
 SVO_HD inline int32_t f2i32(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   // v_cvt_i32_f32 truncates toward zero, clamps out-of-range values to
@@ -238,6 +248,31 @@ SVO_HD inline void dda_step_axes(V3f dir, V3f sig, V3f* p, V3f bmin, float size,
   *ax = m == dx;
   *ay = m == dy;
   *az = m == dz;
+}
+
+// dda_step_axes from the cell's far faces: far = the cell corner the
+// direction points to per axis (min + size * max(signum, 0)). The reference's
+// size*max(s,0) - s*(p - min) equals far - p on every axis when p - min is
+// exact, which holds in the power-of-two layouts (min a multiple of a
+// power-of-two size >= 1, coordinates below 2^24, p within about half a
+// cell of the cell: min = 0, or Sterbenz's lemma, or p - min in [size, 2 size)
+// on the representable grid of size): for s = +1 both are the once-rounded
+// real value min + size - p; for s = -1 the reference's is p - min exactly
+// and far - p its negation, equal after |.|; for s = NaN every scale factor is
+// NaN and both distances are NaN.
+// Returns the step length m: when m is not NaN, at least one axis steps
+// (fminf returns one of its operands, and m == that operand).
+SVO_HD inline float dda_step_far(V3f dir, V3f* p, V3f far, V3f scale, bool* ax, bool* ay,
+                                 bool* az) {
+  const float dx = fabsf((far.x - p->x) * scale.x);
+  const float dy = fabsf((far.y - p->y) * scale.y);
+  const float dz = fabsf((far.z - p->z) * scale.z);
+  const float m = fminf(fminf(dx, dy), dz);
+  *p = *p + dir * m;
+  *ax = m == dx;
+  *ay = m == dy;
+  *az = m == dz;
+  return m;
 }
 
 // Exact reciprocal of a power of two, 0 for any other value. For s = 2^k the

@@ -711,6 +711,20 @@ def current_render_variant() -> int:
     return int(N.lib().svo_current_render_variant())
 
 
+def render_variant_scratch(index: int) -> int:
+    """Scratch bytes per lane of a render variant's build (non-zero: spills)."""
+    v = C.c_uint32(0)
+    _check(N.lib().svo_render_variant_scratch(index, C.byref(v)))
+    return int(v.value)
+
+
+def batch_variant_scratch(index: int) -> int:
+    """Scratch bytes per lane of a batched variant's build (non-zero: spills)."""
+    v = C.c_uint32(0)
+    _check(N.lib().svo_batch_variant_scratch(index, C.byref(v)))
+    return int(v.value)
+
+
 def set_timing(enabled: bool) -> None:
     N.lib().svo_set_timing(int(enabled))
 

@@ -222,6 +222,23 @@ def test_every_render_variant_matches_oracle(gpu_device):
         select_render_variant(before)
 
 
+def test_kernel_builds_report_their_scratch(gpu_device):
+    """Every variant reports its scratch bytes per lane; the initial render
+    variant and the batched variant the serving loop uses do not spill, and
+    most builds do not either (the autotuners consider only those)."""
+    from shocovox_amd import (batch_variant_scratch, render_variant_scratch,
+                              render_variants)
+    _say("scratch bytes of every kernel variant")
+    rn, bn = render_variants(), batch_variants()
+    rs = {n: render_variant_scratch(i) for i, n in enumerate(rn)}
+    bs = {n: batch_variant_scratch(i) for i, n in enumerate(bn)}
+    _say(f"render {rs}\nbatch {bs}")
+    assert rs["w6-slut-8x8-ww"] == 0 and rs["w6-slut-8x8-ww-g1"] == 0
+    assert bs["classic-w5-g5"] == 0
+    assert sum(v == 0 for v in rs.values()) >= len(rs) // 2
+    assert sum(v == 0 for v in bs.values()) >= len(bs) // 2
+
+
 def test_in_place_edits_on_device(gpu_device):
     """Sparse edits reach the device through svo_tree_update (changed records and
     bricks only); each round's render equals the oracle on the edited tree."""
