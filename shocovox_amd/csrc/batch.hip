@@ -112,7 +112,7 @@ __device__ inline const TreeView* trees_of(const char* table) {
 // workgroup keeps those of the tree of the run's first job, loaded once at
 // the start (jobs on other trees read global memory).
 template <int MIN_WAVES, bool WW = false, bool LOD = false, int TW = 8, int TH = 8, int NODES = 0,
-          bool SHARED = false, int DEFER = 0>
+          bool SHARED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters) {
   static_assert(TW * TH == 64, "a tile is one pixel per lane");
@@ -202,9 +202,6 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
           // (the crawl shortcut does not fit the 80 VGPRs of six waves per SIMD)
           r = cast_ray_ww<true, uint64_t, (MIN_WAVES < 6)>(
               tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
-        } else if constexpr (DEFER > 0) {
-          r = cast_ray_deferred<DEFER, true>(tv, tv.ray_lut, o, d,
-                                             static_cast<const NodeRec*>(cache), ntop);
         } else {
           r = cast_ray<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
         }
@@ -291,14 +288,6 @@ const BatchVariant kBatchVariants[] = {
     {"classic-w5-g5-n73", batch_kernel<5, false, false, 8, 8, 73>, kLod8x8, 5, 8, 8},
     {"classic-w4-g4-s585", batch_kernel<4, false, false, 8, 8, 585, true>, kLod8x8, 4, 8, 8},
     {"ww-w6-g6-n73", batch_kernel<6, true, false, 8, 8, 73>, kLod8x8, 6, 8, 8},
-    // deferred brick probes: a wave's brick DDA stops when at most 4 / 8 / 16
-    // lanes still step while twice as many wait
-    {"classic-w5-g5-d4", batch_kernel<5, false, false, 8, 8, 0, false, 4>, kLod8x8, 5, 8, 8},
-    {"classic-w5-g5-d8", batch_kernel<5, false, false, 8, 8, 0, false, 8>, kLod8x8, 5, 8, 8},
-    {"classic-w5-g5-d16", batch_kernel<5, false, false, 8, 8, 0, false, 16>, kLod8x8, 5, 8, 8},
-    {"classic-w4-g4-d8", batch_kernel<4, false, false, 8, 8, 0, false, 8>, kLod8x8, 4, 8, 8},
-    {"classic-w4-g4-d16", batch_kernel<4, false, false, 8, 8, 0, false, 16>, kLod8x8, 4, 8, 8},
-    {"classic-w4-g4", batch_kernel<4>, kLod8x8, 4, 8, 8},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int g_batch_variant = 0;

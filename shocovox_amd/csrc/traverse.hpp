@@ -178,67 +178,28 @@ SVO_HD inline uint32_t bitmap_cell(V3f bp) {
          16u * clamp03(f2i32(floorf(bp.z)));
 }
 
-// Lanes of the wave executing this point (a host build is one lane). The
-// host stand-in for the lanes that wait outside the brick DDA is a full
-// wave, so that host builds take every deferral the device could take.
-SVO_HD inline uint64_t wave_lanes_here() {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __ballot(1);
-#else
-  return 1ull;
-#endif
-}
-SVO_HD inline uint64_t wave_lanes_live() {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __ballot(1);
-#else
-  return ~0ull;
-#endif
-}
-
-// A deferred brick DDA: the cell index x | y << 10 | z << 20, or kNoPending.
-constexpr uint32_t kNoPending = 0xFFFFFFFFu;
-
-// probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252).
-// Returns 0 (miss), 1 (hit) or, with DEFER > 0, 2: the brick DDA stopped
-// early and its cell is in *pend (p holds its position); a later call with
-// the same brick and *pend resumes it exactly where it stopped. The DDA of
-// a wave stops early when at most DEFER of its lanes are still stepping
-// while at least 2 DEFER of the lanes in `live` (the lanes still tracing)
-// wait for them: those lanes then go on with their traversal, and the
-// stopped lanes' steps join the next probes of the wave. Every lane's
-// operations and their order are unchanged, so results are too.
-template <bool P2, int DEFER = 0>
-SVO_HD __attribute__((always_inline)) inline int probe_brick_impl(
-    const TreeView& t, uint32_t form, uint32_t word, Cube bb, V3f d, V3f sig, V3f scale, V3f* p,
-    CastResult* r, uint32_t* pend, uint64_t live) {
-  // (the general form bounds each probe by a step count, which a resumed
-  // probe would restart)
-  static_assert(DEFER == 0 || P2, "deferred probes need the power-of-two form");
-  if (form == FORM_EMPTY) return 0;
+// probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
+template <bool P2>
+SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t, uint32_t form,
+                                                              uint32_t word, Cube bb, V3f d,
+                                                              V3f sig, V3f scale, V3f* p,
+                                                              CastResult* r) {
+  if (form == FORM_EMPTY) return false;
   if (form == FORM_SOLID) {
     r->pix = word;
     r->impact = *p;
     r->cell = bb;
-    return 1;
+    return true;
   }
   const int dim = static_cast<int>(t.dim);
   const float fdim = static_cast<float>(t.dim);
-  int ix, iy, iz;
-  if (DEFER > 0 && *pend != kNoPending) {
-    ix = static_cast<int>(*pend & 1023u);
-    iy = static_cast<int>((*pend >> 10) & 1023u);
-    iz = static_cast<int>((*pend >> 20) & 1023u);
-  } else {
-    const float rcp_size = size_rcp<P2>(bb.size);
-    const V3f rel = (*p - bb.min) * fdim;
-    const V3f pib = div3_size<P2>(rel, bb.size, rcp_size);
-    auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
-    ix = clampi(f2i32(pib.x));
-    iy = clampi(f2i32(pib.y));
-    iz = clampi(f2i32(pib.z));
-  }
-  if constexpr (DEFER > 0) *pend = kNoPending;
+  const float rcp_size = size_rcp<P2>(bb.size);
+  const V3f rel = (*p - bb.min) * fdim;
+  const V3f pib = div3_size<P2>(rel, bb.size, rcp_size);
+  auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
+  int ix = clampi(f2i32(pib.x));
+  int iy = clampi(f2i32(pib.y));
+  int iz = clampi(f2i32(pib.z));
   const float unit = div_size<P2>(bb.size, fdim, t.inv_dim);
   V3f cmin = bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
                           unit;
@@ -302,18 +263,6 @@ SVO_HD __attribute__((always_inline)) inline int probe_brick_impl(
     } else {
       if (--left == 0) break;
     }
-    if constexpr (DEFER > 0) {
-      // only a cell inside the brick is kept (10 bits per axis, dim <= 1024;
-      // a cell outside ends the probe at the next test anyway)
-      const uint64_t here = wave_lanes_here();
-      if (dim <= 1024 && static_cast<uint32_t>(ix | iy | iz) < static_cast<uint32_t>(dim) &&
-          __builtin_popcountll(here) <= DEFER &&
-          __builtin_popcountll(live & ~here) >= 2 * DEFER) {
-        *pend = static_cast<uint32_t>(ix) | (static_cast<uint32_t>(iy) << 10) |
-                (static_cast<uint32_t>(iz) << 20);
-        return 2;
-      }
-    }
   }
   if (occupied) {
     const uint64_t n3 = static_cast<uint64_t>(t.dim) * t.dim * t.dim;
@@ -325,18 +274,10 @@ SVO_HD __attribute__((always_inline)) inline int probe_brick_impl(
                   div_size<P2>(bb.size, fdim, t.inv_dim)};
     r->impact = *p;
     r->cell = hb;
-    return 1;
+    return true;
   }
   if (in) r->flags |= FLAG_ITERATION_GUARD;
-  return 0;
-}
-
-template <bool P2>
-SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t, uint32_t form,
-                                                              uint32_t word, Cube bb, V3f d,
-                                                              V3f sig, V3f scale, V3f* p,
-                                                              CastResult* r) {
-  return probe_brick_impl<P2>(t, form, word, bb, d, sig, scale, p, r, nullptr, 0ull) == 1;
+  return false;
 }
 
 // The rest of a ray whose POP of the root empties the stack because the
@@ -398,16 +339,10 @@ struct RingStack {
 // POP and is not reset by a restart from the root; when the level is below
 // |origin - round(p / (2 level)) * 2 level| / viewing_distance, the node's
 // MIP brick is probed over the node's bounds like a uniform leaf's brick.
-//
-// With DEFER > 0 (plain traversal only), a brick probe may stop early
-// (probe_brick_impl); the lane then skips the rest of its iteration, and its
-// next iteration re-reads the same node record and resumes the probe
-// without counting another step, so it continues exactly where it stopped.
-template <bool LOD, bool P2, class LutT, int DEFER = 0>
+template <bool LOD, bool P2, class LutT>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
     uint32_t ntop, float viewing_distance) {
-  static_assert(!(LOD && DEFER > 0), "deferred probes are for the plain traversal");
   // Only the flags live across the loops; the probe fills pix, impact and
   // cell on the hit path, and a miss returns miss_flags(flags). (Zeroing
   // the hit fields up front would make the compiler carry them around every
@@ -442,13 +377,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
   // log2(size / dim) of two powers of two, exactly
   float mip_level =
       LOD ? static_cast<float>(__builtin_ctz(static_cast<uint32_t>(t.size) / t.dim)) : 0.0f;
-  uint32_t pend = kNoPending;  // a deferred probe of this lane (DEFER > 0)
   // One loop runs the reference's two: an iteration of its stack loop, and
   // when the stack has emptied, first its restart from the root (which ends
   // the ray when the nudged point leaves the tree).
   for (;;) {
-    const uint64_t live = DEFER > 0 ? wave_lanes_live() : 0ull;
-    const bool resume = DEFER > 0 && pend != kNoPending;
     if (stk.count == 0u) {
       // restart from the root (:551-562)
       p = p + d * 0.1f;
@@ -460,8 +392,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
       bounds = root;
       stk.push(0u);
     }
-    // (a resumed iteration was counted when it began)
-    if (!resume && ++steps > t.max_steps) {
+    if (++steps > t.max_steps) {
       r.flags |= FLAG_ITERATION_GUARD;
       break;
     }
@@ -486,12 +417,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     if (target != OOB_OCTANT && (kind == KIND_UNIFORM_LEAF || kind == KIND_LEAF)) {
       const bool uniform = kind == KIND_UNIFORM_LEAF;
       const uint32_t slot = uniform ? 0u : target;
-      const int pr = probe_brick_impl<P2, DEFER>(
-          t, brick_form(meta, slot), slot == target ? tword : n.child[0],
-          uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale, &p, &r, &pend,
-          live);
-      if (pr == 1) goto hit_found;
-      if (DEFER > 0 && pr == 2) continue;
+      if (probe_brick<P2>(t, brick_form(meta, slot), slot == target ? tword : n.child[0],
+                          uniform ? bounds : child_bounds_for(bounds, target), d, sig, scale,
+                          &p, &r))
+        goto hit_found;
     }
     const float rcp_size = size_rcp<P2>(bounds.size);
     const V3f rel = (p - bounds.min) * 4.0f;
@@ -754,14 +683,6 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray(const TreeView&
                                                                 const NodeRec* top = nullptr,
                                                                 uint32_t ntop = 0u) {
   return cast_ray_impl<false, P2>(t, lut, o, d, top, ntop, 0.0f);
-}
-
-// cast_ray with deferred brick probes (cast_ray_impl's DEFER)
-template <int DEFER, bool P2 = false, class LutT>
-SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_deferred(
-    const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top = nullptr,
-    uint32_t ntop = 0u) {
-  return cast_ray_impl<false, P2, LutT, DEFER>(t, lut, o, d, top, ntop, 0.0f);
 }
 
 template <bool P2 = false, class LutT>

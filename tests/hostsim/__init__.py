@@ -47,15 +47,12 @@ def ray_lut() -> np.ndarray:
 
 def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False,
          max_steps: int = 0, viewing_distance: float | None = None,
-         general: bool = False, regrouped: bool | str = False,
-         deferred: bool = False) -> np.ndarray:
+         general: bool = False, regrouped: bool | str = False) -> np.ndarray:
     """The product traversal on the host: cast_ray, with resumable=True its
     ray_begin / ray_step form, with a viewing distance cast_ray_lod, with
     regrouped=True the while-while form cast_ray_ww (regrouped="nocrawl":
-    that form without the crawl shortcut), with deferred=True cast_ray with
-    deferred brick probes (each probe stopped after every step and resumed).
-    As on the device, power-of-two trees take the P2 form unless
-    general=True."""
+    that form without the crawl shortcut). As on the device, power-of-two
+    trees take the P2 form unless general=True."""
     a = flat_arrays
     keep = [np.ascontiguousarray(a["nodes"]), np.ascontiguousarray(a["brick_pix"]),
             np.ascontiguousarray(a["brick_occ"]), np.ascontiguousarray(a["colors"]), ray_lut()]
@@ -67,7 +64,6 @@ def cast(flat_arrays: dict, rays: np.ndarray, resumable: bool = False,
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
     out = np.zeros(len(rays), HIT_DTYPE)
     mode = 2 if viewing_distance is not None else (
-        5 if deferred else
         4 if regrouped == "nocrawl" else (3 if regrouped else int(resumable)))
     lib().hs_cast(C.byref(t), rays.ctypes.data, len(rays), out.ctypes.data, mode, int(general))
     return out

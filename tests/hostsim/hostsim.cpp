@@ -34,11 +34,8 @@ struct hs_hit {
 // mode 0: cast_ray; mode 1: the resumable form (ray_begin, then ray_step
 // until the ray finishes), as the refill kernels run it; mode 2: cast_ray_lod
 // with t->viewing_distance; mode 3: cast_ray_ww; mode 4: cast_ray_ww without
-// the crawl shortcut; mode 5: cast_ray with deferred brick probes (every
-// probe stops after each DDA step and resumes in the next iteration: the
-// host build is one lane with a full wave waiting). Modes 0, 2, 3 and 5 use
-// the P2 form, as the kernels do, when exact_pow2_sizes holds and `general`
-// is 0, else the general form (mode 5: cast_ray's).
+// the crawl shortcut. Modes 0, 2 and 3 use the P2 form, as the kernels
+// do, when exact_pow2_sizes holds and `general` is 0, else the general form.
 void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int mode,
              int general) {
   const bool p2 = !general && svo::exact_pow2_sizes(static_cast<uint64_t>(t->size), t->dim);
@@ -57,9 +54,6 @@ void hs_cast(const hs_tree* t, const float* rays, uint64_t n, hs_hit* out, int m
     } else if (mode == 4) {
       c = p2 ? svo::cast_ray_ww<true, uint64_t, false>(v, v.ray_lut, o, d, nullptr, 0u)
              : svo::cast_ray_ww<false, uint64_t, false>(v, v.ray_lut, o, d, nullptr, 0u);
-    } else if (mode == 5) {
-      c = p2 ? svo::cast_ray_deferred<8, true>(v, v.ray_lut, o, d)
-             : svo::cast_ray<false>(v, o, d);
     } else if (mode == 2) {
       c = p2 ? svo::cast_ray_lod<true>(v, v.ray_lut, o, d, t->viewing_distance)
              : svo::cast_ray_lod<false>(v, v.ray_lut, o, d, t->viewing_distance);

@@ -278,8 +278,6 @@ def test_resumable_traversal_equals_cast_ray():
         # the while-while arrangement (cast_ray_ww) too
         assert base.tobytes() == hostsim.cast(a, rays, regrouped=True).tobytes()
         assert base.tobytes() == hostsim.cast(a, rays, regrouped="nocrawl").tobytes()
-        # and deferred brick probes (each probe stopped after every step)
-        assert base.tobytes() == hostsim.cast(a, rays, deferred=True).tobytes()
         exp = oracle.OracleTree(t.export_model()).cast(rays)
         r = compare(exp, {k: res[k] for k in res.dtype.names})
         assert all(v == 0 for k, v in r.items() if k != "rays"), r
@@ -360,8 +358,6 @@ def test_rays_that_leave_the_content_behind():
             for form in (True, "nocrawl"):
                 g2 = hostsim.cast(a, rays, max_steps=limit, regrouped=form)
                 assert g1.tobytes() == g2.tobytes(), (size, limit, form)
-            g3 = hostsim.cast(a, rays, max_steps=limit, deferred=True)
-            assert g1.tobytes() == g3.tobytes(), (size, limit, "deferred")
 
 
 def test_regrouped_traversal_under_forced_guard():
@@ -379,8 +375,6 @@ def test_regrouped_traversal_under_forced_guard():
         assert base.tobytes() == hostsim.cast(a, rays, max_steps=limit, regrouped=True).tobytes()
         assert base.tobytes() == hostsim.cast(a, rays, max_steps=limit,
                                               regrouped="nocrawl").tobytes()
-        assert base.tobytes() == hostsim.cast(a, rays, max_steps=limit,
-                                              deferred=True).tobytes()
 
 
 def test_forced_iteration_guard_reports_and_ends_rays():
