@@ -334,16 +334,17 @@ uint32_t LayoutMirror::write_brick(const Octree& tree, const Brick& b, uint32_t 
     return slot;  // the same content as the slot's last write
   }
   slot_version_[slot] = b.version;
-  occ_scratch_.assign(words, 0u);
-  for (size_t i = 0; i < n3; ++i)
-    if (!tree.pix_points_to_empty(b.parted[i])) occ_scratch_[i / 32] |= 1u << (i % 32);
-  encode_slot(slot, b.parted.data(), occ_scratch_.data());
+  encode_slot(tree, slot, b.parted.data());
   return slot;
 }
 
-// Puts a slot's new contents into the mirror and lists the cheapest form that
-// brings the device's copy of the slot to them.
-void LayoutMirror::encode_slot(uint32_t slot, const uint32_t* new_pix, const uint32_t* new_occ) {
+// Puts a slot's new contents (and their occupancy sidecar) into the mirror
+// and lists the cheapest form that brings the device's copy of the slot to
+// them. Two passes over the brick: one that builds the sidecar and counts
+// the words that differ from the device's copy and from the two candidate
+// fill values (the first and the last word), one that writes the mirror
+// and emits the chosen form's words.
+void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t* new_pix) {
   const uint32_t dim = flat_.brick_dim;
   const uint32_t n3 = dim * dim * dim;
   const uint32_t words = occ_words_per_brick(dim);
@@ -353,17 +354,35 @@ void LayoutMirror::encode_slot(uint32_t slot, const uint32_t* new_pix, const uin
   // the device holds the mirror's words for a slot it has received and that
   // this patch has not written yet
   const bool known = slot < synced_slots_ && !again;
-  // one pass: words that differ from the device's copy, and from the two
-  // candidate fill values (the first and the last word)
+  occ_scratch_.assign(words, 0u);
+  uint32_t* new_occ = occ_scratch_.data();
   const uint32_t p0 = new_pix[0], p1 = new_pix[n3 - 1];
-  const uint32_t o0 = new_occ[0], o1 = new_occ[words - 1];
-  size_t dk = 0, dp0 = 0, dp1 = 0, do0 = 0, do1 = 0;
-  for (uint32_t i = 0; i < n3; ++i) {
-    const uint32_t a = new_pix[i];
-    dk += a != pix[i];
-    dp0 += a != p0;
-    dp1 += a != p1;
+  size_t dk = 0, dp0 = 0, dp1 = 0;
+  for (uint32_t c = 0; c * 32u < n3; ++c) {
+    // one sidecar word per 32 values; the counts of a chunk vectorize, and a
+    // chunk of empty markers needs no palette lookups
+    const uint32_t* a = new_pix + c * 32u;
+    const uint32_t* o = pix + c * 32u;
+    const uint32_t m = std::min(32u, n3 - c * 32u);
+    uint32_t k = 0, f0 = 0, f1 = 0, empties = 0;
+    for (uint32_t j = 0; j < m; ++j) {
+      k += a[j] != o[j];
+      f0 += a[j] != p0;
+      f1 += a[j] != p1;
+      empties += a[j] == EMPTY_MARKER;
+    }
+    dk += k;
+    dp0 += f0;
+    dp1 += f1;
+    if (empties != m) {
+      uint32_t bits = 0;
+      for (uint32_t j = 0; j < m; ++j)
+        if (!tree.pix_points_to_empty(a[j])) bits |= 1u << j;
+      new_occ[c] = bits;
+    }
   }
+  const uint32_t o0 = new_occ[0], o1 = new_occ[words - 1];
+  size_t do0 = 0, do1 = 0;
   for (uint32_t i = 0; i < words; ++i) {
     const uint32_t a = new_occ[i];
     dk += a != occ[i];
@@ -387,20 +406,35 @@ void LayoutMirror::encode_slot(uint32_t slot, const uint32_t* new_pix, const uin
     rewritten_ = true;
   } else if (whole_cost <= fill_cost && whole_cost <= known_cost) {
     whole_slots_.push_back(slot);
-  } else {
-    const bool fill = fill_cost < known_cost;
-    if (fill) {
-      fills_.push_back(SlotFill{slot, fp, fo});
-      for (uint32_t i = 0; i < n3; ++i)
-        if (new_pix[i] != fp) pix_writes_.push_back(WordWrite{slot, i, new_pix[i]});
-      for (uint32_t i = 0; i < words; ++i)
-        if (new_occ[i] != fo) occ_writes_.push_back(WordWrite{slot, i, new_occ[i]});
-    } else {
-      for (uint32_t i = 0; i < n3; ++i)
-        if (new_pix[i] != pix[i]) pix_writes_.push_back(WordWrite{slot, i, new_pix[i]});
-      for (uint32_t i = 0; i < words; ++i)
-        if (new_occ[i] != occ[i]) occ_writes_.push_back(WordWrite{slot, i, new_occ[i]});
+  } else if (fill_cost < known_cost) {
+    fills_.push_back(SlotFill{slot, fp, fo});
+    for (uint32_t i = 0; i < n3; ++i) {
+      const uint32_t a = new_pix[i];
+      if (a != fp) pix_writes_.push_back(WordWrite{slot, i, a});
+      pix[i] = a;
     }
+    for (uint32_t i = 0; i < words; ++i) {
+      const uint32_t a = new_occ[i];
+      if (a != fo) occ_writes_.push_back(WordWrite{slot, i, a});
+      occ[i] = a;
+    }
+    return;
+  } else {
+    for (uint32_t i = 0; i < n3; ++i) {
+      const uint32_t a = new_pix[i];
+      if (a != pix[i]) {
+        pix_writes_.push_back(WordWrite{slot, i, a});
+        pix[i] = a;
+      }
+    }
+    for (uint32_t i = 0; i < words; ++i) {
+      const uint32_t a = new_occ[i];
+      if (a != occ[i]) {
+        occ_writes_.push_back(WordWrite{slot, i, a});
+        occ[i] = a;
+      }
+    }
+    return;
   }
   std::copy(new_pix, new_pix + n3, pix);
   std::copy(new_occ, new_occ + words, occ);

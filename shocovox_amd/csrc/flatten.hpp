@@ -102,7 +102,7 @@ class LayoutMirror {
   uint32_t record_for(uint32_t key);
   void write_record(const Octree& tree, uint32_t key, std::vector<uint32_t>* pending);
   uint32_t write_brick(const Octree& tree, const Brick& b, uint32_t old_slot);
-  void encode_slot(uint32_t slot, const uint32_t* new_pix, const uint32_t* new_occ);
+  void encode_slot(const Octree& tree, uint32_t slot, const uint32_t* new_pix);
   void drop_partial_forms();
   void release_slot(uint32_t slot);
   void release_key(uint32_t key);
