@@ -252,9 +252,11 @@ int svo_tree_from_octree(svo_octree* tree, svo_tree** out);
 /* Bring a device tree up to date with edits made to the host octree since it
  * was created or last updated (OctreeGPUView::reload,
  * src/raytracing/bevy/mod.rs:56-60): only node records and bricks that
- * changed are uploaded; arrays that must grow are moved to larger ones on the
- * device. Consumes the octree's log of changed nodes, so one host octree
- * feeds one device tree.
+ * changed are uploaded, a changed brick as its changed words, as a fill of
+ * its slot plus the words that differ from the fill, or whole, whichever is
+ * smallest; arrays that must grow are moved to larger ones on the device.
+ * Consumes the octree's log of changed nodes, so one host octree feeds one
+ * device tree.
  * Asynchronous on `stream`: the call returns once the patch is queued (it
  * waits on the host only for the update two calls back, whose pinned staging
  * it reuses), and the host octree may be edited again right away. Casts that

@@ -267,6 +267,66 @@ def test_in_place_edits_on_device(gpu_device):
         assert_parity(exp, gpu_to_numpy(bufs), f"edit round {rnd}")
 
 
+@pytest.mark.parametrize("size,dim,mips", [(32, 2, False), (64, 4, True), (128, 8, False)])
+def test_patch_forms_on_device(gpu_device, size, dim, mips):
+    """In-place patches in all three brick forms reach the device intact:
+    word writes (a few voxels of an existing brick), slot fills plus words
+    (new bricks, Solid bricks split by a clear) and whole slots (bricks of
+    dimension 2 with three or more changed words, whole-node edits), with
+    MIP bricks patched as well on one tree. Every round's render and a set of
+    incoherent rays equal the oracle on the edited tree."""
+    import torch
+    from shocovox_amd import OctreeEntry
+    rng = np.random.default_rng(size + dim)
+    tree = Octree.new(size, dim)
+    if mips:
+        tree.albedo_mip_map_resampling_strategy().switch_albedo_mip_maps(True)
+    xyz = rng.integers(0, size, size=(size * size, 3)).astype(np.uint32)
+    ents = np.zeros((len(xyz), 3), np.uint32)
+    ents[:, 0] = 1
+    ents[:, 1] = (rng.integers(1, 9, len(xyz)).astype(np.uint32) << 24) | 0xFF
+    tree.insert_many(xyz, ents)
+    d = DeviceTree(tree)
+    s = float(size)
+    vp = W.look_at_viewport((1.7 * s, 1.2 * s, 1.5 * s), (s / 2, s / 2, s / 2), 256, 160)
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, 256, 160)
+    inc = W.incoherent_rays(size, 4000, seed=size + 3)
+    for rnd in range(6):
+        kind = rnd % 3
+        if kind == 0:  # sparse voxels: word writes, new bricks
+            n = int(rng.integers(50, 600))
+            xyz = rng.integers(0, size, size=(n, 3)).astype(np.uint32)
+            ents = np.zeros((n, 3), np.uint32)
+            ents[:, 0] = 1
+            ents[:, 1] = (rng.integers(1, 9, n).astype(np.uint32) << 24) | 0x7F
+            tree.insert_many(xyz, ents)
+            tree.clear_many(rng.integers(0, size, size=(n // 2, 3)).astype(np.uint32))
+        elif kind == 1:  # whole nodes: solid regions appear and split
+            for _ in range(4):
+                p = tuple(int(v) for v in rng.integers(0, size, size=3))
+                tree.insert_at_lod(p, int(rng.choice([2, 4, 8])), OctreeEntry.visual(0x3366AAFF))
+            tree.clear_many(rng.integers(0, size, size=(200, 3)).astype(np.uint32))
+        else:
+            for _ in range(3):
+                p = tuple(int(v) for v in rng.integers(0, size, size=3))
+                tree.clear_at_lod(p, int(rng.choice([2, 4])))
+        recs, slots = d.update(tree)
+        _say(f"patch forms {size}^3 dim {dim} round {rnd}: {recs} records, {slots} slots")
+        got = gpu_to_numpy(d.render(vp, 256, 160))
+        torch.cuda.synchronize()
+        o = _oracle(tree)
+        assert_parity(o.cast(rays), got, f"render after round {rnd}")
+        assert_parity(o.cast(inc), _cast_tree(d, inc, gpu_device), f"rays after round {rnd}")
+
+
+def _cast_tree(d, rays, dev) -> dict:
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(rays, np.float32).reshape(-1, 6)).to(dev)
+    bufs = d.cast(t)
+    torch.cuda.synchronize()
+    return gpu_to_numpy(bufs)
+
+
 def test_patch_that_outgrows_the_device_arrays(gpu_device):
     """Edits that need more records, brick slots and palette entries than the
     device arrays hold grow them on the device (old contents copied device to
