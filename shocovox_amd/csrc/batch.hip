@@ -205,6 +205,24 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
         } else {
           r = cast_ray<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
         }
+        // The origin and the output index are formed again for the store
+        // rather than carried through the traversal (four registers fewer
+        // there): the empty asm hides that `local` is unchanged, so the
+        // compiler recomputes them (a render's origin is the job's camera,
+        // a cast's is read again from its ray).
+        uint32_t again = local;
+        asm volatile("" : "+s"(again));
+        if (job.kind == 0) {
+          const uint32_t image_tile = again + job.first_tile;
+          const uint32_t x = (image_tile % job.tiles_x) * TW + (lane % TW);
+          const uint32_t y = (image_tile / job.tiles_x) * TH + (lane / TW);
+          o = job.cam.origin;
+          idx = job.tile_major ? again * 64u + lane : y * job.width + x;
+        } else {
+          idx = again * 64u + lane;
+          const float* rr = job.rays + 6ull * idx;
+          o = V3f{rr[0], rr[1], rr[2]};
+        }
         store_result(tv, job.out, idx, r, o);
       }
     }
@@ -288,6 +306,9 @@ const BatchVariant kBatchVariants[] = {
     {"classic-w5-g5-n73", batch_kernel<5, false, false, 8, 8, 73>, kLod8x8, 5, 8, 8},
     {"classic-w4-g4-s585", batch_kernel<4, false, false, 8, 8, 585, true>, kLod8x8, 4, 8, 8},
     {"ww-w6-g6-n73", batch_kernel<6, true, false, 8, 8, 73>, kLod8x8, 6, 8, 8},
+    // six waves per SIMD (at most 80 VGPRs)
+    {"classic-w6-g6", batch_kernel<6>, kLod8x8, 6, 8, 8},
+    {"classic-w6-g6-n73", batch_kernel<6, false, false, 8, 8, 73>, kLod8x8, 6, 8, 8},
 };
 constexpr int kBatchVariantCount = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int g_batch_variant = 0;

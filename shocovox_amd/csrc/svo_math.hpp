@@ -53,6 +53,8 @@ SVO_HD inline float rsignum(float x) {
   if (x != x) return x;
   return std::signbit(x) ? -1.0f : 1.0f;
 }
+// rsignum for a non-NaN x (+-1 by the sign bit); +-1 for NaN as well
+SVO_HD inline float sign_one(float x) { return copysignf(1.0f, x); }
 
 // `f as i32` (saturating, NaN -> 0)
 // Bit (b mod 32) of w.

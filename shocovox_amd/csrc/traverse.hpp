@@ -352,7 +352,6 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
 
   const V3f scale = dda_scale_factors(d);
   const uint32_t dir_idx = hash_direction(d);
-  const V3f sig{rsignum(d.x), rsignum(d.y), rsignum(d.z)};
   const Cube root{v3u(0.0f), t.size};
   const float lo = 0.00001f;
   const float hi = 4.0f - 0.00001f;
@@ -381,6 +380,17 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
   // when the stack has emptied, first its restart from the root (which ends
   // the ray when the nudged point leaves the tree).
   for (;;) {
+    // signum(d) per axis, formed again in every iteration from d's sign bits
+    // (one instruction each) rather than kept in three registers: the empty
+    // asm makes d opaque here, so the compiler cannot hoist these (or other
+    // values derived from d) out of the loop. For a NaN component the
+    // reference's signum is NaN and this one +-1; then every scale factor is
+    // NaN (each involves all three components), every DDA distance is NaN
+    // and no step selects a signum, so the difference is never observed.
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
+#endif
+    const V3f sig{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
     if (stk.count == 0u) {
       // restart from the root (:551-562)
       p = p + d * 0.1f;
@@ -534,7 +544,6 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
 
   const V3f scale = dda_scale_factors(d);
   const uint32_t dir_idx = hash_direction(d);
-  const V3f sig{rsignum(d.x), rsignum(d.y), rsignum(d.z)};
   const Cube root{v3u(0.0f), t.size};
   const float lo = 0.00001f;
   const float hi = 4.0f - 0.00001f;
@@ -562,6 +571,11 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     // phase 1: stack-loop iterations up to the next due probe
     bool due = false;
     while (!finished) {
+      // signum(d), formed in each iteration as in cast_ray_impl
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
+#endif
+      const V3f sig{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
       if (stk.count == 0u) {
         // restart from the root (:551-562)
         p = p + d * 0.1f;
@@ -659,6 +673,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     if (!due) continue;
     // phase 2: the due probe of the iteration, lanes of the wave together
     {
+#if defined(__HIP_DEVICE_COMPILE__)
+      asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
+#endif
+      const V3f sig{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
       const NodeRec& n = (stk.s0 < ntop ? top : t.nodes)[stk.s0];
       const uint32_t meta = n.meta;
       const bool uniform = node_kind(meta) == KIND_UNIFORM_LEAF;
