@@ -199,9 +199,7 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
           else
             r = cast_ray<true>(tv, o, d);
         } else if constexpr (WW) {
-          // (the crawl shortcut does not fit the 80 VGPRs of six waves per SIMD)
-          r = cast_ray_ww<true, uint64_t, (MIN_WAVES < 6)>(
-              tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
+          r = cast_ray_ww<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
         } else {
           r = cast_ray<true>(tv, tv.ray_lut, o, d, static_cast<const NodeRec*>(cache), ntop);
         }

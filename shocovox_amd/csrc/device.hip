@@ -72,10 +72,9 @@ __global__ __launch_bounds__(256) void render_lod_kernel(TreeView t, Camera cam,
 // each of the 8 XCDs (which receive workgroups round-robin in dispatch
 // order) renders one contiguous band of the image, keeping that band's nodes
 // and bricks in its own L2. WW runs the while-while arrangement of the
-// traversal (cast_ray_ww), which probes the due bricks of a wave together;
-// its builds for six waves per SIMD leave out the crawl shortcut (CRAWL).
+// traversal (cast_ray_ww), which probes the due bricks of a wave together.
 template <int MIN_WAVES, bool LDS_LUT, int TW, int TH, int NODES = 0, bool XCD = false,
-          bool WW = false, bool G1 = false, bool CRAWL = MIN_WAVES < 6>
+          bool WW = false, bool G1 = false>
 __global__ __launch_bounds__(G1 ? 64 : 256) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) {
   static_assert(TW * TH == 64, "a wave covers 64 pixels");
@@ -113,7 +112,7 @@ void render_kernel(TreeView t, Camera cam, uint32_t w, uint32_t h, HitPtrs out) 
   const V3f d = camera_ray_dir(cam, x, y);
   CastResult res;
   if constexpr (WW)
-    res = cast_ray_ww<true, uint64_t, CRAWL>(t, lut, cam.origin, d, s_nodes, ntop);
+    res = cast_ray_ww<true>(t, lut, cam.origin, d, s_nodes, ntop);
   else
     res = cast_ray<true>(t, lut, cam.origin, d, s_nodes, ntop);
   store_result(t, out, y * w + x, res, cam.origin);

@@ -532,8 +532,7 @@ hit_found:
 // the wave's lanes wait for their probes, measured slower.)
 //
 // CRAWL = false leaves out the crawl_out_of_root shortcut (the restarts are
-// then run one by one, with the same result): it needs three more VGPRs,
-// which builds held to 80 (six waves per SIMD) do not have.
+// then run one by one, with the same result; the host tests run both forms).
 template <bool P2, class LutT, bool CRAWL = true>
 SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
     const TreeView& t, const LutT* __restrict__ lut, V3f o, V3f d, const NodeRec* top,
