@@ -380,6 +380,11 @@ bool validate_desc(const svo_tree_desc& d, std::string* err) {
     *err = "brick arrays missing";
     return false;
   }
+  if (static_cast<uint64_t>(d.brick_count) * occ_words_per_brick(d.brick_dim) * 4u >
+      OCC_RANGE_MAX) {
+    *err = "occupancy sidecar beyond the 4 GiB the traversal's 32-bit offsets address";
+    return false;
+  }
   for (uint32_t i = 0; i < d.node_count; ++i) {
     const svo_node_rec& r = d.nodes[i];
     const uint32_t kind = r.meta & 3u;
@@ -453,6 +458,11 @@ namespace {
 // Capacity an upload allocates for `need` elements: a quarter of headroom so
 // that edits which add nodes or bricks are patched in place.
 uint64_t grown_capacity(uint64_t need) { return need + need / 4 + 1; }
+// Slots of an occupancy allocation: `want`, or fewer (but at least `need`)
+// so that the allocation stays within the traversal's 32-bit read offsets.
+uint64_t occ_slots(uint64_t want, uint64_t need, uint64_t slot_bytes) {
+  return std::max(need, std::min(want, OCC_RANGE_MAX / slot_bytes));
+}
 // Palettes grow with almost every edit that brings a new colour, and an entry
 // costs 4 bytes, so they get a fixed allowance on top.
 uint64_t grown_color_capacity(uint64_t need) { return grown_capacity(need) + 4096; }
@@ -504,7 +514,9 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   auto grow_bytes = [&](uint32_t** p, uint64_t* cap_bytes, uint64_t need,
                         uint64_t slot_bytes) -> int {
     if (need <= *cap_bytes && *p) return SVO_OK;
-    const uint64_t nb = grown_capacity(slots) * slot_bytes;
+    const uint64_t nb = (p == &t->brick_occ ? occ_slots(grown_capacity(slots), slots, slot_bytes)
+                                            : grown_capacity(slots)) *
+                        slot_bytes;
     void* np = nullptr;
     hipError_t ge = hipMalloc(&np, nb);
     if (ge != hipSuccess) return hip_fail(ge, "hipMalloc(bricks)", err);
@@ -565,6 +577,7 @@ int device_tree_upload(DeviceTree* t, const svo_tree_desc& d, void* stream, std:
   t->view.node_count = d.node_count;
   t->view.max_steps = t->step_limit ? t->step_limit : MAX_TRAVERSAL_STEPS;
   t->view.mips_enabled = d.mips_enabled;
+  t->view.occ_range = static_cast<uint32_t>(t->occ_bytes);
   return SVO_OK;
 }
 
@@ -655,8 +668,9 @@ int grow_resident(DeviceTree* t, const svo_tree_desc& d, hipStream_t s, std::str
                 t->brick_count * n3 * 4, n3 * 4, s, err);
     if (st == SVO_OK) t->pix_bytes = pix_cap * n3 * 4;
     if (st == SVO_OK)
-      st = regrow(t, reinterpret_cast<void**>(&t->brick_occ), &occ_cap, nc,
-                  t->brick_count * words * 4, words * 4, s, err);
+      st = regrow(t, reinterpret_cast<void**>(&t->brick_occ), &occ_cap,
+                  occ_slots(nc, d.brick_count, words * 4), t->brick_count * words * 4, words * 4, s,
+                  err);
     if (st == SVO_OK) t->occ_bytes = occ_cap * words * 4;
     t->brick_capacity = std::min(t->pix_bytes / (n3 * 4), t->occ_bytes / (words * 4));
   }
@@ -670,6 +684,7 @@ int grow_resident(DeviceTree* t, const svo_tree_desc& d, hipStream_t s, std::str
   t->view.brick_pix = t->brick_pix;
   t->view.brick_occ = t->brick_occ;
   t->view.colors = t->colors;
+  t->view.occ_range = static_cast<uint32_t>(t->occ_bytes);
   return SVO_OK;
 }
 

@@ -33,6 +33,9 @@ namespace svo {
 constexpr uint32_t FLAG_ITERATION_GUARD = 1u;
 constexpr uint32_t FLAG_BITMAP_RANGE = 2u;
 constexpr uint32_t MAX_TRAVERSAL_STEPS = 1u << 22;
+// largest occupancy allocation (bytes) of a device tree: its reads take
+// 32-bit byte offsets
+constexpr uint64_t OCC_RANGE_MAX = 0xFFFFFFFFull;
 
 struct TreeView {
   const NodeRec* nodes;
@@ -48,7 +51,29 @@ struct TreeView {
   uint32_t node_count;
   uint32_t max_steps;  // iteration bound of one ray (MAX_TRAVERSAL_STEPS unless lowered)
   uint32_t mips_enabled;  // MIPMapStrategy::enabled: LOD queries probe node MIPs
+  // bytes of the device's brick_occ allocation (below 2^32): the range of
+  // its 32-bit offset reads (occ_word); 0 on the host, where it is unused
+  uint32_t occ_range;
 };
+
+// Occupancy word flat >> 5 of the brick whose words begin at byte offset
+// brick_off of brick_occ, for a probe whose cell index may lie outside the
+// brick (`in` false: the caller then ignores the word). On the device the
+// read is a buffer load whose range is the whole occupancy allocation, so
+// such an index reads some word of the allocation, or 0 past its end, and
+// needs no clamp; on the host it reads the brick's first word instead.
+SVO_HD inline uint32_t occ_word(const TreeView& t, uint32_t brick_off, uint32_t flat, bool in) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  (void)in;
+  // 0x00020000: 32-bit data format, the word-read descriptor of gfx9 parts
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint32_t*>(t.brick_occ), 0, static_cast<int>(t.occ_range), 0x00020000);
+  return __builtin_amdgcn_raw_buffer_load_b32(rsrc, static_cast<int>(brick_off + ((flat >> 5) << 2)),
+                                              0, 0);
+#else
+  return t.brick_occ[(brick_off >> 2) + (in ? (flat >> 5) : 0u)];
+#endif
+}
 
 // Every bounds size a traversal meets is the tree size halved k times (PUSH)
 // or doubled back (POP), never below the brick dimension, and a brick's
@@ -208,11 +233,6 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   // leaves the others (their step component is +0.0, and cell corners are
   // never -0.0): the per-axis moves are fixed for the whole probe.
   const V3f umove{sig.x * unit, sig.y * unit, sig.z * unit};
-  const int mx = step_int(sig.x), my = step_int(sig.y), mz = step_int(sig.z);
-  // The loop has one exit: the cell is outside the brick, or occupied, or
-  // the step bound is reached (then `in` still holds and `occupied` not).
-  // The occupancy word is read for every cell, from index 0 when the cell
-  // is outside, so that the test needs no branch.
   const uint32_t shift = static_cast<uint32_t>(__builtin_ctz(t.dim));
   // ix + iy dim + iz dim^2 (P2: the factors as shifts)
   auto flat_of = [&](int x, int y, int z) -> uint32_t {
@@ -223,44 +243,67 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
       return static_cast<uint32_t>(x + y * dim + z * dim * dim);
     }
   };
-  if constexpr (P2)
+  // The loop has one exit: the cell is outside the brick, or occupied, or
+  // (without P2) the step bound is reached (then `in` still holds and
+  // `occupied` not). The occupancy word is read for every cell, from index 0
+  // when the cell is outside, so that the test needs no branch.
+  bool in, occupied;
+  if constexpr (P2) {
+    // Cell coordinates mirrored on the axes whose signum is -1: a = ix
+    // (signum +1) or dim-1-ix = ix ^ (dim-1) (signum -1), so that every move
+    // is +1 and an axis leaves the brick at a = dim; the cell's flat index is
+    // the mirrored one xor the mirror mask. With dim a power of two, all
+    // three are below dim exactly when their bitwise or is.
+    const uint32_t dm1 = t.dim - 1u;
+    const uint32_t nx = sig.x < 0.0f ? dm1 : 0u;
+    const uint32_t ny = sig.y < 0.0f ? dm1 : 0u;
+    const uint32_t nz = sig.z < 0.0f ? dm1 : 0u;
+    const uint32_t mirror = nx | (ny << shift) | (nz << (2u * shift));
+    uint32_t ax = static_cast<uint32_t>(ix) ^ nx;
+    uint32_t ay = static_cast<uint32_t>(iy) ^ ny;
+    uint32_t az = static_cast<uint32_t>(iz) ^ nz;
+    const uint32_t occ_off = word * t.occ_words * 4u;
+    // cmin holds the cell's far corner (the corner the direction points to)
     cmin = V3f{cmin.x + unit * rmax(sig.x, 0.0f), cmin.y + unit * rmax(sig.y, 0.0f),
                cmin.z + unit * rmax(sig.z, 0.0f)};
-  bool in, occupied;
-  for (int left = 4 * dim + 16;;) {
-    if constexpr (P2) {
-      // with dim a power of two, all three are in [0, dim) exactly when
-      // their bitwise or is (a negative index has the top bit set): one
-      // test instead of six
-      in = static_cast<uint32_t>(ix | iy | iz) < static_cast<uint32_t>(dim);
-    } else {
-      in = ix >= 0 && ix < dim && iy >= 0 && iy < dim && iz >= 0 && iz < dim;
-    }
-    const uint32_t flat = in ? flat_of(ix, iy, iz) : 0u;
-    const uint32_t bits = occ[flat >> 5];
-    occupied = in & (bit_of(bits, flat) != 0u);
-    if (!in | occupied) break;
-    bool sx, sy, sz;
-    float m = 0.0f;
-    if constexpr (P2) {
-      // cmin holds the cell's far corner here (set before the loop)
-      m = dda_step_far(d, p, cmin, scale, &sx, &sy, &sz);
-    } else {
-      dda_step_axes(d, sig, p, cmin, unit, scale, &sx, &sy, &sz);
-    }
-    cmin = V3f{sx ? cmin.x + umove.x : cmin.x, sy ? cmin.y + umove.y : cmin.y,
-               sz ? cmin.z + umove.z : cmin.z};
-    ix += sx ? mx : 0;
-    iy += sy ? my : 0;
-    iz += sz ? mz : 0;
-    if constexpr (P2) {
+    for (;;) {
+      in = (ax | ay | az) < t.dim;
+      const uint32_t flat = (ax | (ay << shift) | (az << (2u * shift))) ^ mirror;
+      const uint32_t bits = occ_word(t, occ_off, flat, in);
+      occupied = in & (bit_of(bits, flat) != 0u);
+      if (!in | occupied) break;
+      bool sx, sy, sz;
+      const float m = dda_step_far(d, p, cmin, scale, &sx, &sy, &sz);
+      cmin = V3f{sx ? cmin.x + umove.x : cmin.x, sy ? cmin.y + umove.y : cmin.y,
+                 sz ? cmin.z + umove.z : cmin.z};
+      ax += static_cast<uint32_t>(sx);
+      ay += static_cast<uint32_t>(sy);
+      az += static_cast<uint32_t>(sz);
       // A step of length m (not NaN) moves at least one axis, each axis only
       // in its direction's sign (signum is +-1 there: a NaN direction makes
       // every distance NaN), so the cell leaves the brick within 3 dim
       // steps. A NaN step moves nothing (p becomes NaN and stays so), so the
-      // loop would repeat that state: it ends here, as after the step bound.
+      // loop would repeat that state: it ends here.
       if (!(m == m)) break;
-    } else {
+    }
+    ix = static_cast<int>(ax ^ nx);
+    iy = static_cast<int>(ay ^ ny);
+    iz = static_cast<int>(az ^ nz);
+  } else {
+    const int mx = step_int(sig.x), my = step_int(sig.y), mz = step_int(sig.z);
+    for (int left = 4 * dim + 16;;) {
+      in = ix >= 0 && ix < dim && iy >= 0 && iy < dim && iz >= 0 && iz < dim;
+      const uint32_t flat = in ? flat_of(ix, iy, iz) : 0u;
+      const uint32_t bits = occ[flat >> 5];
+      occupied = in & (bit_of(bits, flat) != 0u);
+      if (!in | occupied) break;
+      bool sx, sy, sz;
+      dda_step_axes(d, sig, p, cmin, unit, scale, &sx, &sy, &sz);
+      cmin = V3f{sx ? cmin.x + umove.x : cmin.x, sy ? cmin.y + umove.y : cmin.y,
+                 sz ? cmin.z + umove.z : cmin.z};
+      ix += sx ? mx : 0;
+      iy += sy ? my : 0;
+      iz += sz ? mz : 0;
       if (--left == 0) break;
     }
   }
