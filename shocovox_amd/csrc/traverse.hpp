@@ -434,17 +434,6 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
 #endif
     const V3f sig{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
-    if (stk.count == 0u) {
-      // restart from the root (:551-562)
-      p = p + d * 0.1f;
-      target = ((p.x < t.size) & (p.y < t.size) & (p.z < t.size) & (p.x > 0.0f) & (p.y > 0.0f) &
-                 (p.z > 0.0f))
-                   ? hash_region(p, t.size / 2.0f)
-                   : OOB_OCTANT;
-      if (target == OOB_OCTANT) break;
-      bounds = root;
-      stk.push(0u);
-    }
     if (++steps > t.max_steps) {
       r.flags |= FLAG_ITERATION_GUARD;
       break;
@@ -489,72 +478,99 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
     // reads: the same decision as testing the child first)
     const bool push = !pop && kind == KIND_INTERNAL &&
                       (occ & octant_bitmap_mask(target)) != 0ull && tword != INVALID_INDEX;
+    // PUSH, POP and ADVANCE leave the next bounds and target here, and the
+    // stack moves once after them, by selects: the loop has a single back
+    // edge, so its state stays in the same registers on every path.
+    Cube next_bounds = bounds;
+    uint32_t next_target = target;
     if (push) {
       // PUSH (:484-492)
-      bounds = tb;
-      target = hash_region(p - tb.min, tb.size / 2.0f);
-      stk.push(tword);
-      if (LOD) mip_level -= 1.0f;
-      continue;
-    }
-    // POP (:445-475) and ADVANCE (:497-544) both begin with one DDA step,
-    // out of the node (POP, unless the stack empties) or out of the target
-    // child (ADVANCE); it is issued once for the lanes of both. (The POP's
-    // other values depend on the bounds only, so taking the step before
-    // them changes nothing.)
-    if (!pop && ++steps > t.max_steps) {
-      r.flags |= FLAG_ITERATION_GUARD;
-      break;
-    }
-    // (left unset when the POP empties the stack: then it is not read)
-    V3f st;
-    if (!pop || stk.count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
-    if (pop) {
-      if (stk.count == 1u && key == 0u && !backtrack && target != OOB_OCTANT &&
-          kind != KIND_LEAF && !(LOD && t.mips_enabled))
-        return crawl_out_of_root(t, p, d, steps, r.flags);
-      stk.pop();
-      if (LOD) mip_level += 1.0f;
-      if (stk.count != 0u) {
-        const V3f center = bounds.min + v3u(bounds.size / 2.0f);
-        const float s2 = bounds.size * 2.0f;
-        const float rcp_s2 = size_rcp<P2>(s2);
-        const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
-                       sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
-                       sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
-        const uint32_t from = hash_region(center - pmin, bounds.size);
-        target = step_octant(from, st);
-        bounds.size *= 2.0f;
-        bounds.min = pmin;
-      }
-      continue;
-    }
-    // the rest of ADVANCE
-    bool guard = false;
-    target = step_octant(target, st);
-    bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
-    while (target != OOB_OCTANT) {
-      tb = child_bounds_for(bounds, target);
-      const V3f sd = st * 4.0f;
-      bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
-      flat = bitmap_index(bp, &r.flags);
-      // stop at a reachable occupied child or a non-empty leaf brick; the
-      // tests run cheapest first and the child word is read last
-      if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
-      if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
-          (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[target] != INVALID_INDEX)
-        break;
-      if (++steps > t.max_steps) {
-        guard = true;
+      next_bounds = tb;
+      next_target = hash_region(p - tb.min, tb.size / 2.0f);
+    } else {
+      // POP (:445-475) and ADVANCE (:497-544) both begin with one DDA step,
+      // out of the node (POP, unless the stack empties) or out of the target
+      // child (ADVANCE); it is issued once for the lanes of both. (The POP's
+      // other values depend on the bounds only, so taking the step before
+      // them changes nothing.)
+      if (!pop && ++steps > t.max_steps) {
+        r.flags |= FLAG_ITERATION_GUARD;
         break;
       }
-      st = dda_step(d, sig, &p, tb, scale);
-      target = step_octant(target, st);
+      // (left unset when the POP empties the stack: then it is not read)
+      V3f st;
+      if (!pop || stk.count > 1u) st = dda_step(d, sig, &p, pop ? bounds : tb, scale);
+      if (pop) {
+        if (stk.count == 1u && key == 0u && !backtrack && target != OOB_OCTANT &&
+            kind != KIND_LEAF && !(LOD && t.mips_enabled))
+          return crawl_out_of_root(t, p, d, steps, r.flags);
+        if (stk.count > 1u) {
+          // the parent's bounds and the sibling the step leads to
+          const V3f center = bounds.min + v3u(bounds.size / 2.0f);
+          const float s2 = bounds.size * 2.0f;
+          const float rcp_s2 = size_rcp<P2>(s2);
+          const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
+                         sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
+                         sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
+          const uint32_t from = hash_region(center - pmin, bounds.size);
+          next_target = step_octant(from, st);
+          next_bounds = Cube{pmin, s2};
+        } else {
+          // the POP empties the stack: the reference's restart from the
+          // root (:551-562), which ends the ray once the nudged point has
+          // left the tree (taken here, not at the next iteration's start:
+          // nothing happens in between)
+          p = p + d * 0.1f;
+          next_target = ((p.x < t.size) & (p.y < t.size) & (p.z < t.size) & (p.x > 0.0f) &
+                         (p.y > 0.0f) & (p.z > 0.0f))
+                            ? hash_region(p, t.size / 2.0f)
+                            : OOB_OCTANT;
+          if (next_target == OOB_OCTANT) break;
+          next_bounds = root;
+        }
+      } else {
+        // the rest of ADVANCE
+        bool guard = false;
+        next_target = step_octant(target, st);
+        bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
+        while (next_target != OOB_OCTANT) {
+          tb = child_bounds_for(bounds, next_target);
+          const V3f sd = st * 4.0f;
+          bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
+          flat = bitmap_index(bp, &r.flags);
+          // stop at a reachable occupied child or a non-empty leaf brick; the
+          // tests run cheapest first and the child word is read last
+          if (kind == KIND_LEAF && brick_form(meta, next_target) != FORM_EMPTY) break;
+          if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(next_target)) != 0ull &&
+              (lut[flat * 8u + dir_idx] & occ) != 0ull && n.child[next_target] != INVALID_INDEX)
+            break;
+          if (++steps > t.max_steps) {
+            guard = true;
+            break;
+          }
+          st = dda_step(d, sig, &p, tb, scale);
+          next_target = step_octant(next_target, st);
+        }
+        if (guard) {
+          r.flags |= FLAG_ITERATION_GUARD;
+          break;
+        }
+      }
     }
-    if (guard) {
-      r.flags |= FLAG_ITERATION_GUARD;
-      break;
-    }
+    // PUSH shifts the stack down (its oldest entry drops out once four are
+    // held), POP shifts it up, and a POP that empties it is followed by the
+    // restart's push of the root; ADVANCE leaves it
+    const uint32_t s0 = stk.s0, s1 = stk.s1, s2 = stk.s2, s3 = stk.s3;
+    const bool emptied = pop && stk.count == 1u;
+    stk.s0 = push ? tword : (pop ? (emptied ? 0u : s1) : s0);
+    stk.s1 = push ? s0 : (pop ? s2 : s1);
+    stk.s2 = push ? s1 : (pop ? s3 : s2);
+    stk.s3 = push ? s2 : s3;
+    stk.count = push ? (stk.count < 4u ? stk.count + 1u : 4u)
+                     : (pop ? (emptied ? 1u : stk.count - 1u) : stk.count);
+    if (LOD) mip_level += push ? -1.0f : (pop ? 1.0f : 0.0f);
+    bounds = next_bounds;
+    target = next_target;
   }
   return miss_flags(r.flags);
 hit_found:

@@ -66,6 +66,23 @@ def test_update_that_changes_the_brick_dimension(gpu_device):
         del d, flat
 
 
+def test_occupancy_beyond_the_32_bit_range_is_refused(gpu_device):
+    """The traversal reads occupancy words through 32-bit byte offsets, so a
+    tree whose occupancy sidecar would exceed 4 GiB is refused at upload,
+    before anything is allocated or copied (the description below claims
+    2^24 bricks of 16^3: 8 GiB of occupancy; its arrays are never read)."""
+    from shocovox_amd import OctreeError
+    flat = Octree.scene(2, 64, 16, 3).flatten()
+    d = flat.desc()
+    d.brick_count = 1 << 24
+    _say("upload of a description with 2^24 bricks of 16^3 (8 GiB of occupancy)")
+    with pytest.raises(OctreeError, match="4 GiB"):
+        DeviceTree(desc=d)
+    d = flat.desc()
+    ok = DeviceTree(desc=d)  # the same tree at its real size is accepted
+    del ok, flat
+
+
 def test_batched_pool_at_configured_size(gpu_device):
     """The batched workload at its configured size: 64 trees log-uniform over
     64^3..1024^3 (brick dims 4/8/16, heightfield or caves content), 1-4
