@@ -259,7 +259,10 @@ int svo_tree_from_octree(svo_octree* tree, svo_tree** out);
  * device tree.
  * Asynchronous on `stream`: the call returns once the patch is queued (it
  * waits on the host only for the update two calls back, whose pinned staging
- * it reuses), and the host octree may be edited again right away. Casts that
+ * it reuses; a host function queued on `stream` after the patch's copies
+ * releases the staging, so no event is kept on `stream` and it may be
+ * destroyed after the call), and the host octree may be edited again right
+ * away. Casts that
  * must see the update run on `stream` after it or wait for an event recorded
  * there; casts still running on other streams must have finished before the
  * patch runs, which the caller orders (as `&mut self` does in the
@@ -381,7 +384,10 @@ typedef struct svo_job {
  * order: a run waits, on its stream, for the context's previous run, and the
  * host blocks only when it would refill the job table of the run two calls
  * back before that run's copy has finished. Separate contexts may run
- * concurrently on different streams. */
+ * concurrently on different streams. The context keeps an event recorded on
+ * each of its last two runs' streams: such a stream must not be destroyed
+ * before the context's second-next run or svo_batch_destroy (HIP does not
+ * support waiting on an event last recorded on a destroyed stream). */
 int svo_batch_create(uint32_t max_jobs, svo_batch** out);
 void svo_batch_destroy(svo_batch* batch);
 /* Launches all jobs as one persistent kernel on `stream` (asynchronous). The

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -696,12 +697,7 @@ namespace {
 // has finished).
 int reserve_set(DeviceTree* t, DeviceTree::PatchStaging& ps, size_t bytes, std::string* err) {
   hipError_t e;
-  if (!ps.done) {
-    hipEvent_t ev = nullptr;
-    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-    if (e != hipSuccess) return hip_fail(e, "hipEventCreate(staging)", err);
-    ps.done = ev;
-  }
+  ps.tree = t;
   if (bytes <= ps.dev_bytes && bytes <= ps.host_bytes) return SVO_OK;
   const size_t cap = std::max<size_t>(bytes + bytes / 2, size_t(8) << 20);
   if (bytes > ps.dev_bytes) {
@@ -729,12 +725,21 @@ int reserve_set(DeviceTree* t, DeviceTree::PatchStaging& ps, size_t bytes, std::
 }
 
 // Waits (on the host) until staging set `ps` is no longer read by a patch.
-int drain_set(DeviceTree::PatchStaging& ps, std::string* err) {
-  if (!ps.pending) return SVO_OK;
-  const hipError_t e = hipEventSynchronize(static_cast<hipEvent_t>(ps.done));
-  if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize(staging)", err);
-  ps.pending = false;
+int drain_set(DeviceTree* t, DeviceTree::PatchStaging& ps, std::string*) {
+  std::unique_lock<std::mutex> g(t->staging_mu);
+  t->staging_cv.wait(g, [&] { return !ps.pending; });
   return SVO_OK;
+}
+
+// Host function enqueued after a patch's copies: its staging set is free.
+void staging_released(void* arg) {
+  DeviceTree::PatchStaging* ps = static_cast<DeviceTree::PatchStaging*>(arg);
+  DeviceTree* t = ps->tree;
+  {
+    std::lock_guard<std::mutex> g(t->staging_mu);
+    ps->pending = false;
+  }
+  t->staging_cv.notify_all();
 }
 
 }  // namespace
@@ -744,7 +749,7 @@ int device_tree_reserve_staging(DeviceTree* t, size_t bytes, std::string* err) {
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
   for (DeviceTree::PatchStaging& ps : t->staging) {
     if (bytes > ps.dev_bytes || bytes > ps.host_bytes) {
-      const int st = drain_set(ps, err);
+      const int st = drain_set(t, ps, err);
       if (st != SVO_OK) return st;
     }
     const int st = reserve_set(t, ps, bytes, err);
@@ -808,7 +813,7 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   // the set the patch two calls back used: its copy and scatter must have
   // finished before the host refills it
   DeviceTree::PatchStaging& ps = t->staging[t->next_staging];
-  int sst = drain_set(ps, err);
+  int sst = drain_set(t, ps, err);
   if (sst == SVO_OK) sst = reserve_set(t, ps, bytes, err);
   if (sst != SVO_OK) return sst;
   char* h = static_cast<char*>(ps.host);
@@ -883,10 +888,22 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
                        hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(colors)", err);
   }
-  // no wait here: the set stays pending until the stream passes this event
-  e = hipEventRecord(static_cast<hipEvent_t>(ps.done), s);
-  if (e != hipSuccess) return hip_fail(e, "hipEventRecord(staging)", err);
-  ps.pending = true;
+  // no wait here: the set stays pending until the stream has run the copies
+  // and then staging_released
+  {
+    std::lock_guard<std::mutex> g(t->staging_mu);
+    ps.pending = true;
+  }
+  e = hipLaunchHostFunc(s, staging_released, &ps);
+  if (e != hipSuccess) {
+    // the copies are queued but nothing will report their end: wait here
+    (void)hipStreamSynchronize(s);
+    {
+      std::lock_guard<std::mutex> g(t->staging_mu);
+      ps.pending = false;
+    }
+    return hip_fail(e, "hipLaunchHostFunc(staging)", err);
+  }
   t->next_staging ^= 1u;
   t->node_count = d.node_count;
   t->brick_count = d.brick_count;
@@ -901,10 +918,17 @@ void device_tree_destroy(DeviceTree* t) {
   if (!t) return;
   if (t->device >= 0) (void)hipSetDevice(t->device);
   (void)hipDeviceSynchronize();
+  {
+    // the host functions of the last patches may still be on their way
+    // (bounded: a stream in error may never call them)
+    std::unique_lock<std::mutex> g(t->staging_mu);
+    (void)t->staging_cv.wait_for(g, std::chrono::seconds(5), [&] {
+      return !t->staging[0].pending && !t->staging[1].pending;
+    });
+  }
   for (DeviceTree::PatchStaging& ps : t->staging) {
     if (ps.host) (void)hipHostFree(ps.host);
     if (ps.dev) (void)hipFree(ps.dev);
-    if (ps.done) (void)hipEventDestroy(static_cast<hipEvent_t>(ps.done));
   }
   for (void* p : t->retired) (void)hipFree(p);
   if (t->nodes) (void)hipFree(t->nodes);

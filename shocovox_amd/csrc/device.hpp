@@ -1,7 +1,9 @@
 // Device residency and kernel launches of the traversal path.
 #pragma once
 
+#include <condition_variable>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -34,15 +36,21 @@ struct DeviceTree {
   // one (pinned host buffer, device buffer) while the copy and scatter of the
   // previous patch may still be reading the other; `done` marks the end of
   // the last patch that used a set.
+  // A staging set is pending from the patch that fills it until that patch's
+  // copies have run: a host function enqueued after them on the caller's
+  // stream clears the flag (no event is kept on the caller's stream, which
+  // may be destroyed before the set is reused).
   struct PatchStaging {
     void* host = nullptr;
     uint64_t host_bytes = 0;
     void* dev = nullptr;
     uint64_t dev_bytes = 0;
-    void* done = nullptr;  // hipEvent_t
-    bool pending = false;
+    DeviceTree* tree = nullptr;  // the callback's argument: this set's owner
+    bool pending = false;        // guarded by staging_mu
   };
   PatchStaging staging[2];
+  std::mutex staging_mu;
+  std::condition_variable staging_cv;
   uint32_t next_staging = 0;
   std::vector<void*> retired;    // outgrown device buffers, freed with the tree
   uint64_t retired_bytes = 0;

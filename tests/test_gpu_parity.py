@@ -267,6 +267,45 @@ def test_in_place_edits_on_device(gpu_device):
         assert_parity(exp, gpu_to_numpy(bufs), f"edit round {rnd}")
 
 
+def test_patches_on_streams_destroyed_after_each_update(gpu_device):
+    """svo_tree_update keeps nothing on the caller's stream: every patch below
+    is queued on a fresh HIP stream that is destroyed right after the call,
+    and the next patch (which reuses the staging of the one two calls back)
+    still runs; the final render equals the oracle on the edited tree."""
+    import ctypes
+    import torch
+    torch.cuda.synchronize()
+    # the HIP runtime already in the process (the one libsvo and torch use)
+    with open("/proc/self/maps") as f:
+        paths = sorted({ln.split()[-1] for ln in f if "libamdhip64.so" in ln})
+    assert len(paths) == 1, paths
+    hip = ctypes.CDLL(paths[0])
+    cfg = W.PLAIN
+    tree = W.build_scene(cfg, seed=8)
+    vp = W.viewports(cfg)[0]
+    d = DeviceTree(tree)
+    rng = np.random.default_rng(23)
+    for rnd in range(5):
+        xyz = rng.integers(0, cfg.size, size=(200, 3)).astype(np.uint32)
+        ents = np.zeros((200, 3), np.uint32)
+        ents[:, 0] = 1
+        ents[:, 1] = 0xFF20A0FF
+        if rnd % 2:
+            tree.clear_many(xyz)
+        else:
+            tree.insert_many(xyz, ents)
+        s = ctypes.c_void_p()
+        assert hip.hipStreamCreate(ctypes.byref(s)) == 0
+        _say(f"patch {rnd} on a new stream, then the stream is destroyed")
+        d.update(tree, stream=s.value)
+        assert hip.hipStreamDestroy(s) == 0
+    torch.cuda.synchronize()
+    bufs = d.render(vp, cfg.width, cfg.height)
+    torch.cuda.synchronize()
+    rays = oracle.viewport_rays(vp.origin, vp.direction, vp.frustum, vp.fov, cfg.width, cfg.height)
+    assert_parity(_oracle(tree).cast(rays), gpu_to_numpy(bufs), "patches on destroyed streams")
+
+
 @pytest.mark.parametrize("size,dim,mips", [(32, 2, False), (64, 4, True), (128, 8, False)])
 def test_patch_forms_on_device(gpu_device, size, dim, mips):
     """In-place patches in all three brick forms reach the device intact:
