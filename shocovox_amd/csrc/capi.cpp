@@ -526,7 +526,8 @@ namespace svo {
 static_assert(sizeof(LayoutMirror::SlotFill) == 12 && sizeof(LayoutMirror::WordWrite) == 12,
               "patch triples are three packed u32");
 
-int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_event) {
+int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_event,
+                void* const* end_events, uint32_t n_end) {
   if (!dtree || !tree) return fail(SVO_INVALID_ARGUMENT, "NULL argument");
   std::string err;
   LayoutMirror& m = dtree->mirror;
@@ -551,11 +552,13 @@ int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_eve
     bp.occ_writes = reinterpret_cast<const uint32_t*>(m.occ_writes().data());
     bp.n_occ = m.occ_writes().size();
     st = device_tree_patch(dtree->dev, d, m.dirty_records(), bp, m.colors_synced(), stream,
-                           start_event, &err);
+                           start_event, end_events, n_end, &err);
   }
   if (st == 1) {
     st = start_event ? device_record_event(start_event, stream, &err) : SVO_OK;
     if (st == SVO_OK) st = device_tree_upload(dtree->dev, d, stream, &err);
+    for (uint32_t i = 0; i < n_end && st == SVO_OK; ++i)
+      st = device_record_event(end_events[i], stream, &err);
   }
   if (st != SVO_OK) return fail(st, err);
   m.mark_synced();

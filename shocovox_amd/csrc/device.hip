@@ -766,7 +766,7 @@ int device_record_event(void* event, void* stream, std::string* err) {
 
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<uint32_t>& records,
                       const BrickPatch& bp, uint32_t colors_from, void* stream, void* start_event,
-                      std::string* err) {
+                      void* const* end_events, uint32_t n_end, std::string* err) {
   if (d.brick_dim != t->view.dim) return 1;  // caller re-uploads everything
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipError_t e = hipSetDevice(t->device);
@@ -887,6 +887,10 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
     e = hipMemcpyAsync(t->colors + colors_from, base + o_colors, new_colors * 4u,
                        hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(colors)", err);
+  }
+  for (uint32_t i = 0; i < n_end; ++i) {
+    e = hipEventRecord(static_cast<hipEvent_t>(end_events[i]), s);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord(patch end)", err);
   }
   // no wait here: the set stays pending until the stream has run the copies
   // and then staging_released

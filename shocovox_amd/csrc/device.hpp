@@ -94,11 +94,13 @@ struct BrickPatch {
 // copied device to device on `stream` first, so a patch never re-sends the
 // whole tree from the host. `start_event` (a hipEvent_t, or NULL) is
 // recorded on `stream` before the patch's first device operation, after its
-// host work. Returns 1 (asking for a full upload) only when the brick
-// dimension differs.
+// host work; the `n_end` events of `end_events` right after its last one
+// (ahead of the host function that releases the staging). Returns 1 (asking
+// for a full upload) only when the brick dimension differs.
 int device_tree_patch(DeviceTree* t, const svo_tree_desc& desc,
                       const std::vector<uint32_t>& records, const BrickPatch& bricks,
-                      uint32_t colors_from, void* stream, void* start_event, std::string* err);
+                      uint32_t colors_from, void* stream, void* start_event,
+                      void* const* end_events, uint32_t n_end, std::string* err);
 
 // hipEventRecord(event, stream) with the status mapped.
 int device_record_event(void* event, void* stream, std::string* err);

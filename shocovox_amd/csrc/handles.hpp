@@ -32,6 +32,8 @@ namespace svo {
 void set_last_error(const std::string& msg);
 // svo_tree_update that also records `start_event` (a hipEvent_t, or NULL) on
 // `stream` when the host part of the update is done, before its first device
-// operation.
-int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_event);
+// operation, and the `n_end` events of `end_events` when its last device
+// operation is done.
+int tree_update(svo_tree* dtree, svo_octree* tree, void* stream, void* start_event,
+                void* const* end_events = nullptr, uint32_t n_end = 0);
 }  // namespace svo

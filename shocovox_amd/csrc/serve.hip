@@ -404,14 +404,13 @@ void edit_worker_main(Session* s) {
       } else {
         // the layout update and the patch's staging are host work; job.start
         // marks the start of the patch's device work
-        if (st == SVO_OK) st = tree_update(s->dtrees[q.tree], s->trees[q.tree], stream, job.start);
+        // the patch records job.done and the tree's ready event right after
+        // its copies (ahead of its staging's release on the host)
+        void* const ends[2] = {job.done, s->tree_ready[q.tree]};
+        if (st == SVO_OK)
+          st = tree_update(s->dtrees[q.tree], s->trees[q.tree], stream, job.start, ends, 2);
         if (st == SVO_OK) (void)svo_tree_update_stats(s->dtrees[q.tree], &recs, &bricks);
         err = st == SVO_OK ? "" : svo_last_error();
-        if (st == SVO_OK && (hipEventRecord(job.done, stream) != hipSuccess ||
-                             hipEventRecord(s->tree_ready[q.tree], stream) != hipSuccess)) {
-          st = SVO_HIP_ERROR;
-          err = "edit event record failed";
-        }
       }
       et.applied = s->now();
     }
