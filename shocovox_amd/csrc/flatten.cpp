@@ -334,8 +334,54 @@ uint32_t LayoutMirror::write_brick(const Octree& tree, const Brick& b, uint32_t 
     return slot;  // the same content as the slot's last write
   }
   slot_version_[slot] = b.version;
-  encode_slot(tree, slot, b.parted.data());
+  deferred_.emplace_back(slot, b.parted.data());
   return slot;
+}
+
+// The slots written by an apply, encoded once each, on several threads when
+// they hold many values: each worker takes a contiguous run of them and
+// fills its own lists, which are then appended in run order (the same
+// patch as one thread would make). Workers touch disjoint slots of the
+// mirror and of the dirty flags.
+void LayoutMirror::encode_deferred(const Octree& tree) {
+  std::vector<std::pair<uint32_t, const uint32_t*>> work;
+  {
+    std::vector<uint8_t> seen(flat_.brick_count(), 0);
+    for (size_t i = deferred_.size(); i-- > 0;)
+      if (!seen[deferred_[i].first]) {
+        seen[deferred_[i].first] = 1;
+        work.push_back(deferred_[i]);
+      }
+    std::reverse(work.begin(), work.end());
+    deferred_.clear();
+  }
+  if (work.empty()) return;
+  const size_t n3 = static_cast<size_t>(flat_.brick_dim) * flat_.brick_dim * flat_.brick_dim;
+  size_t workers = 1;
+  if (work.size() * n3 >= (size_t(1) << 20))
+    workers = std::min<size_t>({work.size(), std::max(1u, std::thread::hardware_concurrency()),
+                                size_t(8)});
+  std::vector<EncodeOut> outs(workers);
+  auto run = [&](size_t w) {
+    const size_t begin = work.size() * w / workers, end = work.size() * (w + 1) / workers;
+    for (size_t i = begin; i < end; ++i) encode_slot(tree, work[i].first, work[i].second, &outs[w]);
+  };
+  if (workers == 1) {
+    run(0);
+  } else {
+    std::vector<std::thread> threads;
+    for (size_t w = 1; w < workers; ++w) threads.emplace_back(run, w);
+    run(0);
+    for (std::thread& t : threads) t.join();
+  }
+  for (EncodeOut& o : outs) {
+    dirty_slots_.insert(dirty_slots_.end(), o.dirty.begin(), o.dirty.end());
+    whole_slots_.insert(whole_slots_.end(), o.whole.begin(), o.whole.end());
+    fills_.insert(fills_.end(), o.fills.begin(), o.fills.end());
+    pix_writes_.insert(pix_writes_.end(), o.pix.begin(), o.pix.end());
+    occ_writes_.insert(occ_writes_.end(), o.occ.begin(), o.occ.end());
+    rewritten_ = rewritten_ || o.rewritten;
+  }
 }
 
 // Puts a slot's new contents (and their occupancy sidecar) into the mirror
@@ -344,7 +390,8 @@ uint32_t LayoutMirror::write_brick(const Octree& tree, const Brick& b, uint32_t 
 // the words that differ from the device's copy and from the two candidate
 // fill values (the first and the last word), one that writes the mirror
 // and emits the chosen form's words.
-void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t* new_pix) {
+void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t* new_pix,
+                               EncodeOut* out) {
   const uint32_t dim = flat_.brick_dim;
   const uint32_t n3 = dim * dim * dim;
   const uint32_t words = occ_words_per_brick(dim);
@@ -354,8 +401,8 @@ void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t
   // the device holds the mirror's words for a slot it has received and that
   // this patch has not written yet
   const bool known = slot < synced_slots_ && !again;
-  occ_scratch_.assign(words, 0u);
-  uint32_t* new_occ = occ_scratch_.data();
+  out->scratch.assign(words, 0u);
+  uint32_t* new_occ = out->scratch.data();
   const uint32_t p0 = new_pix[0], p1 = new_pix[n3 - 1];
   size_t dk = 0, dp0 = 0, dp1 = 0;
   for (uint32_t c = 0; c * 32u < n3; ++c) {
@@ -392,7 +439,7 @@ void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t
   if (known && dk == 0) return;
   if (!again) {
     slot_dirty_flag_[slot] = 1;
-    dirty_slots_.push_back(slot);
+    out->dirty.push_back(slot);
   }
   const uint32_t fp = dp0 <= dp1 ? p0 : p1, fo = do0 <= do1 ? o0 : o1;
   const size_t d_fill = std::min(dp0, dp1) + std::min(do0, do1);
@@ -402,20 +449,20 @@ void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t
   const size_t known_cost = known ? dk * w : SIZE_MAX;
   if (again) {
     // its earlier form in this patch is superseded: send it whole
-    whole_slots_.push_back(slot);
-    rewritten_ = true;
+    out->whole.push_back(slot);
+    out->rewritten = true;
   } else if (whole_cost <= fill_cost && whole_cost <= known_cost) {
-    whole_slots_.push_back(slot);
+    out->whole.push_back(slot);
   } else if (fill_cost < known_cost) {
-    fills_.push_back(SlotFill{slot, fp, fo});
+    out->fills.push_back(SlotFill{slot, fp, fo});
     for (uint32_t i = 0; i < n3; ++i) {
       const uint32_t a = new_pix[i];
-      if (a != fp) pix_writes_.push_back(WordWrite{slot, i, a});
+      if (a != fp) out->pix.push_back(WordWrite{slot, i, a});
       pix[i] = a;
     }
     for (uint32_t i = 0; i < words; ++i) {
       const uint32_t a = new_occ[i];
-      if (a != fo) occ_writes_.push_back(WordWrite{slot, i, a});
+      if (a != fo) out->occ.push_back(WordWrite{slot, i, a});
       occ[i] = a;
     }
     return;
@@ -423,14 +470,14 @@ void LayoutMirror::encode_slot(const Octree& tree, uint32_t slot, const uint32_t
     for (uint32_t i = 0; i < n3; ++i) {
       const uint32_t a = new_pix[i];
       if (a != pix[i]) {
-        pix_writes_.push_back(WordWrite{slot, i, a});
+        out->pix.push_back(WordWrite{slot, i, a});
         pix[i] = a;
       }
     }
     for (uint32_t i = 0; i < words; ++i) {
       const uint32_t a = new_occ[i];
       if (a != occ[i]) {
-        occ_writes_.push_back(WordWrite{slot, i, a});
+        out->occ.push_back(WordWrite{slot, i, a});
         occ[i] = a;
       }
     }
@@ -567,6 +614,7 @@ void LayoutMirror::apply(Octree& tree) {
     done[k] = 1;
     write_record(tree, k, &pending);
   }
+  encode_deferred(tree);
   if (rewritten_) drop_partial_forms();
 }
 

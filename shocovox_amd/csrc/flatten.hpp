@@ -101,8 +101,17 @@ class LayoutMirror {
  private:
   uint32_t record_for(uint32_t key);
   void write_record(const Octree& tree, uint32_t key, std::vector<uint32_t>* pending);
+  // What the encoding of some slots adds to the patch (one per worker).
+  struct EncodeOut {
+    std::vector<uint32_t> dirty, whole;
+    std::vector<SlotFill> fills;
+    std::vector<WordWrite> pix, occ;
+    std::vector<uint32_t> scratch;
+    bool rewritten = false;
+  };
   uint32_t write_brick(const Octree& tree, const Brick& b, uint32_t old_slot);
-  void encode_slot(const Octree& tree, uint32_t slot, const uint32_t* new_pix);
+  void encode_slot(const Octree& tree, uint32_t slot, const uint32_t* new_pix, EncodeOut* out);
+  void encode_deferred(const Octree& tree);
   void drop_partial_forms();
   void release_slot(uint32_t slot);
   void release_key(uint32_t key);
@@ -116,7 +125,9 @@ class LayoutMirror {
   std::vector<uint32_t> whole_slots_;
   std::vector<SlotFill> fills_;
   std::vector<WordWrite> pix_writes_, occ_writes_;
-  std::vector<uint32_t> occ_scratch_;
+  // slots written by this apply and the values they take, encoded at its
+  // end (the last write of a slot wins)
+  std::vector<std::pair<uint32_t, const uint32_t*>> deferred_;
   bool rewritten_ = false;  // a slot was written twice in one patch (sent whole)
   bool full_ = true;
   uint32_t colors_synced_ = 0;

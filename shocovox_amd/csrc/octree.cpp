@@ -72,10 +72,22 @@ inline const uint32_t* homogeneous(const Brick& b) {
       return nullptr;
     case BrickKind::Solid:
       return &b.solid;
-    case BrickKind::Parted:
-      for (uint32_t v : b.parted)
-        if (v != b.parted[0]) return nullptr;
+    case BrickKind::Parted: {
+      // the values xor the first, or-ed over blocks of 64 (a reduction the
+      // compiler vectorizes) with one test per block
+      const uint32_t* p = b.parted.data();
+      const size_t n = b.parted.size();
+      const uint32_t f = p[0];
+      size_t i = 0;
+      for (; i + 64 <= n; i += 64) {
+        uint32_t diff = 0;
+        for (size_t j = 0; j < 64; ++j) diff |= p[i + j] ^ f;
+        if (diff) return nullptr;
+      }
+      for (; i < n; ++i)
+        if (p[i] != f) return nullptr;
       return &b.parted[0];
+    }
   }
   return nullptr;
 }

@@ -306,14 +306,17 @@ def test_patches_on_streams_destroyed_after_each_update(gpu_device):
     assert_parity(_oracle(tree).cast(rays), gpu_to_numpy(bufs), "patches on destroyed streams")
 
 
-@pytest.mark.parametrize("size,dim,mips", [(32, 2, False), (64, 4, True), (128, 8, False)])
+@pytest.mark.parametrize("size,dim,mips", [(32, 2, False), (64, 4, True), (128, 8, False),
+                                           (256, 16, False)])
 def test_patch_forms_on_device(gpu_device, size, dim, mips):
     """In-place patches in all three brick forms reach the device intact:
     word writes (a few voxels of an existing brick), slot fills plus words
     (new bricks, Solid bricks split by a clear) and whole slots (bricks of
     dimension 2 with three or more changed words, whole-node edits), with
     MIP bricks patched as well on one tree. Every round's render and a set of
-    incoherent rays equal the oracle on the edited tree."""
+    incoherent rays equal the oracle on the edited tree. (At 256^3 and dim 16
+    a round's bricks hold over 2^20 values: the layout encodes them on
+    several threads.)"""
     import torch
     from shocovox_amd import OctreeEntry
     rng = np.random.default_rng(size + dim)

@@ -212,12 +212,14 @@ def _random_edits(tree, rng, n_insert, n_clear, size, colors=6):
     tree.clear_many(rng.integers(0, size, size=(n_clear, 3)).astype(np.uint32))
 
 
-@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("seed", range(5))
 def test_incremental_layout_matches_fresh_tree(seed):
     """Edits applied in place to the layout (svo_flat_sync) answer casts and
-    point queries exactly like the edited tree (oracle) after every round."""
+    point queries exactly like the edited tree (oracle) after every round.
+    (256^3 at dim 16: a round's bricks hold over 2^20 values, so their
+    encoding runs on several threads.)"""
     rng = np.random.default_rng(seed)
-    size, dim = [(32, 2), (64, 4), (128, 8), (64, 1)][seed]
+    size, dim = [(32, 2), (64, 4), (128, 8), (64, 1), (256, 16)][seed]
     tree = Octree.new(size, dim)
     _random_edits(tree, rng, 2000, 0, size)
     flat = tree.flatten()
