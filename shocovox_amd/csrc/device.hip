@@ -773,6 +773,10 @@ int device_tree_patch(DeviceTree* t, const svo_tree_desc& d, const std::vector<u
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice", err);
   const uint32_t n3 = d.brick_dim * d.brick_dim * d.brick_dim;
   const uint32_t words = occ_words_per_brick(d.brick_dim);
+  if (static_cast<uint64_t>(d.brick_count) * words * 4u > OCC_RANGE_MAX) {
+    *err = "occupancy sidecar beyond the 4 GiB the traversal's 32-bit offsets address";
+    return SVO_INVALID_ARGUMENT;
+  }
   for (uint32_t r : records)
     if (r >= d.node_count) {
       *err = "patched record out of range";
