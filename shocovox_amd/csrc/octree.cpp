@@ -515,8 +515,12 @@ uint64_t Octree::brick_calculate_occupied_bits(const Brick& b) const {
 
 // BrickData::simplify (node.rs:316-331)
 bool Octree::brick_simplify(Brick& b) const {
+  if (b.kind == BrickKind::Parted && b.version != 0 && b.mixed_at == b.version) return false;
   const uint32_t* h = homogeneous(b);
-  if (!h) return false;
+  if (!h) {
+    if (b.kind == BrickKind::Parted) b.mixed_at = b.version;
+    return false;
+  }
   const uint32_t v = *h;
   if (pix_points_to_empty(v))
     b = Brick::empty();
@@ -952,46 +956,58 @@ bool Octree::simplify(uint32_t key) {
       }
       const uint32_t dim = brick_dim_;
       const uint64_t half = dim / 2;
-      std::vector<uint32_t> unified(static_cast<size_t>(dim) * dim * dim, EMPTY_MARKER);
+      // The eight bricks merge into one brick of the leaf's size when every
+      // Empty or Solid brick equals the first brick and every Parted brick
+      // is uniform in each of its 2x2x2 blocks. That is tested first, and the
+      // merged brick (each octant a half-size copy) is built only then.
+      auto octant_off = [half](uint32_t o, uint64_t off[3]) {
+        const V3f f = octant_offset(o) * static_cast<float>(half);
+        off[0] = round_usize(f.x);
+        off[1] = round_usize(f.y);
+        off[2] = round_usize(f.z);
+      };
+      auto block_uniform = [&](const std::vector<uint32_t>& p, uint64_t x, uint64_t y,
+                               uint64_t z) {
+        const uint32_t v = p[flat_projection(x * 2, y * 2, z * 2, dim)];
+        return v == p[flat_projection(x * 2 + 1, y * 2, z * 2, dim)] &&
+               v == p[flat_projection(x * 2, y * 2 + 1, z * 2, dim)] &&
+               v == p[flat_projection(x * 2, y * 2, z * 2 + 1, dim)] &&
+               v == p[flat_projection(x * 2 + 1, y * 2 + 1, z * 2, dim)] &&
+               v == p[flat_projection(x * 2, y * 2 + 1, z * 2 + 1, dim)] &&
+               v == p[flat_projection(x * 2 + 1, y * 2, z * 2 + 1, dim)] &&
+               v == p[flat_projection(x * 2 + 1, y * 2 + 1, z * 2 + 1, dim)];
+      };
       bool uniform = true;
       for (uint32_t o = 0; o < 8 && uniform; ++o) {
-        uint64_t off[3];
-        {
-          const V3f f = octant_offset(o) * static_cast<float>(half);
-          off[0] = round_usize(f.x);
-          off[1] = round_usize(f.y);
-          off[2] = round_usize(f.z);
-        }
         const Brick& b = n.bricks[o];
-        if (b.kind == BrickKind::Empty) {
-          uniform = uniform && (b == n.bricks[0]);
-        } else if (b.kind == BrickKind::Solid) {
-          uniform = uniform && (b == n.bricks[0]);
-          for (uint64_t x = off[0]; x < off[0] + half; ++x)
-            for (uint64_t y = off[1]; y < off[1] + half; ++y)
-              for (uint64_t z = off[2]; z < off[2] + half; ++z)
-                unified[flat_projection(x, y, z, dim)] = b.solid;
+        if (b.kind != BrickKind::Parted) {
+          uniform = b == n.bricks[0];
         } else {
-          const std::vector<uint32_t>& p = b.parted;
           for (uint64_t x = 0; x < half && uniform; ++x)
             for (uint64_t y = 0; y < half && uniform; ++y)
-              for (uint64_t z = 0; z < half && uniform; ++z) {
-                const uint32_t v = p[flat_projection(x * 2, y * 2, z * 2, dim)];
-                if (v == p[flat_projection(x * 2 + 1, y * 2, z * 2, dim)] &&
-                    v == p[flat_projection(x * 2, y * 2 + 1, z * 2, dim)] &&
-                    v == p[flat_projection(x * 2, y * 2, z * 2 + 1, dim)] &&
-                    v == p[flat_projection(x * 2 + 1, y * 2 + 1, z * 2, dim)] &&
-                    v == p[flat_projection(x * 2, y * 2 + 1, z * 2 + 1, dim)] &&
-                    v == p[flat_projection(x * 2 + 1, y * 2, z * 2 + 1, dim)] &&
-                    v == p[flat_projection(x * 2 + 1, y * 2 + 1, z * 2 + 1, dim)]) {
-                  unified[flat_projection(off[0] + x, off[1] + y, off[2] + z, dim)] = v;
-                } else {
-                  uniform = false;
-                }
-              }
+              for (uint64_t z = 0; z < half && uniform; ++z)
+                uniform = block_uniform(b.parted, x, y, z);
         }
       }
       if (uniform) {
+        std::vector<uint32_t> unified(static_cast<size_t>(dim) * dim * dim, EMPTY_MARKER);
+        for (uint32_t o = 0; o < 8; ++o) {
+          uint64_t off[3];
+          octant_off(o, off);
+          const Brick& b = n.bricks[o];
+          if (b.kind == BrickKind::Solid) {
+            for (uint64_t x = off[0]; x < off[0] + half; ++x)
+              for (uint64_t y = off[1]; y < off[1] + half; ++y)
+                for (uint64_t z = off[2]; z < off[2] + half; ++z)
+                  unified[flat_projection(x, y, z, dim)] = b.solid;
+          } else if (b.kind == BrickKind::Parted) {
+            for (uint64_t x = 0; x < half; ++x)
+              for (uint64_t y = 0; y < half; ++y)
+                for (uint64_t z = 0; z < half; ++z)
+                  unified[flat_projection(off[0] + x, off[1] + y, off[2] + z, dim)] =
+                      b.parted[flat_projection(x * 2, y * 2, z * 2, dim)];
+          }
+        }
         n = NodeContent::uniform(Brick::make_parted(std::move(unified)));
         simplified = true;
       }

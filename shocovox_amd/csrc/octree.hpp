@@ -41,6 +41,9 @@ struct Brick {
   // takes a fresh one through touch()). 0 = unknown. The layout mirror uses
   // it to skip bricks an edit did not change; equality ignores it.
   uint64_t version = 0;
+  // The version at which brick_simplify last found the values not all
+  // equal (0: never): the same content needs no second scan.
+  mutable uint64_t mixed_at = 0;
 
   static Brick empty() { return Brick{}; }
   static Brick make_solid(uint32_t v) {
