@@ -542,7 +542,7 @@ def spheres_block(args) -> dict:
     ref = [{k: b[k].clone() for k in fields} for b in (pr, pc)]
     names = batch_variants()
     tuning = {}
-    for name in ("classic-w5-g5", "ww-w5-g5", "ww-w6-g6", "classic-w5-g5-n73"):
+    for name in ("classic-w5-g5", "classic-w6-g6", "ww-w5-g5", "ww-w6-g6", "classic-w5-g5-n73"):
         if name not in names:
             continue
         select_batch_variant(names.index(name))

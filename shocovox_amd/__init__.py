@@ -9,13 +9,14 @@ from .octree import (  # noqa: F401
     OctreeEntry, OctreeError, Viewport, albedo, albedo_rgba, alloc_hits, batch_variant_scratch,
     batch_variants, collect_kernel_ms, current_batch_variant, current_render_variant,
     render_variant_scratch, render_variants, select_batch_variant, select_render_variant,
-    set_timing, voxel_data,
+    set_cast_order_threshold, set_timing, voxel_data,
 )
 
 __all__ = [
     "COMPLEX", "EMPTY", "EMPTY_MARKER", "INFORMATIVE", "VISUAL", "Batch", "DeviceTree", "FlatTree",
     "MIPResamplingMethods", "Octree", "StrategyUpdater", "OctreeEntry", "OctreeError", "Viewport", "albedo", "albedo_rgba", "alloc_hits",
     "batch_variant_scratch", "batch_variants", "collect_kernel_ms", "current_batch_variant", "current_render_variant",
-    "render_variant_scratch", "render_variants", "select_batch_variant", "select_render_variant", "set_timing",
+    "render_variant_scratch", "render_variants", "select_batch_variant", "select_render_variant",
+    "set_cast_order_threshold", "set_timing",
     "voxel_data",
 ]

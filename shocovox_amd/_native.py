@@ -181,6 +181,7 @@ _SIGS = {
     "svo_current_render_variant": (C.c_int, []),
     "svo_render_variant_scratch": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
     "svo_set_timing": (C.c_int, [C.c_int]),
+    "svo_cast_set_order_threshold": (C.c_int, [_U32]),
     "svo_collect_kernel_ms": (_U32, [C.POINTER(C.c_float), _U32]),
 }
 

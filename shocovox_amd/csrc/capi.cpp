@@ -613,6 +613,11 @@ int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hit
   return SVO_OK;
 }
 
+int svo_cast_set_order_threshold(uint32_t n) {
+  svo::set_cast_order_threshold(n);
+  return SVO_OK;
+}
+
 int svo_render(const svo_tree* dtree, const svo_viewport* vp, uint32_t w, uint32_t h,
                const svo_hits* out, void* stream) {
   if (!image_fits(w, h)) return fail(SVO_INVALID_ARGUMENT, kImageTooLarge);

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -21,10 +22,14 @@ namespace svo {
 
 namespace {
 
+// `order` (optional): lane i traverses ray order[i] (svo_cast's traversal
+// order for large ray sets, order.hip); each ray writes its own outputs.
 __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __restrict__ rays,
-                                                   uint32_t n, HitPtrs out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                                                   uint32_t n, const uint32_t* __restrict__ order,
+                                                   HitPtrs out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = order ? order[k] : k;
   const float* r = rays + 6ull * i;
   const V3f o{r[0], r[1], r[2]};
   const V3f d{r[3], r[4], r[5]};
@@ -956,16 +961,31 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
+  uint32_t* order = nullptr;
+  const uint32_t threshold = cast_order_threshold();
+  if (threshold != 0 && n >= threshold) {
+    const int st = order_rays(rays, n, t.view.size, &order, s, err);
+    if (st != SVO_OK) return st;
+  }
   // n < 2^32, so the grid (at most 2^24 workgroups) is computed without
   // wrapping in 64 bits
   const uint32_t block = 256;
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(n) + block - 1) / block);
-  hipLaunchKernelGGL(cast_kernel, dim3(grid), dim3(block), 0, s, t.view, rays, n, to_ptrs(out));
+  hipLaunchKernelGGL(cast_kernel, dim3(grid), dim3(block), 0, s, t.view, rays, n, order,
+                     to_ptrs(out));
   hipError_t e = hipGetLastError();
+  if (order) (void)hipFreeAsync(order, s);
   if (e != hipSuccess) return hip_fail(e, "cast_kernel launch", err);
   timer.done();
   return SVO_OK;
 }
+
+namespace {
+std::atomic<uint32_t> g_cast_order_threshold{1u << 20};
+}  // namespace
+
+void set_cast_order_threshold(uint32_t n) { g_cast_order_threshold.store(n); }
+uint32_t cast_order_threshold() { return g_cast_order_threshold.load(); }
 
 int device_cast_lod(const DeviceTree& t, const float* rays, uint32_t n, float viewing_distance,
                     const svo_hits& out, void* stream, std::string* err) {

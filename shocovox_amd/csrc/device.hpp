@@ -107,6 +107,15 @@ int device_record_event(void* event, void* stream, std::string* err);
 
 int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hits& out,
                 void* stream, std::string* err);
+// svo_cast's traversal order (order.hip): *order receives n ray indices sorted
+// by (Morton code of the origin on a 64^3 grid of a tree of size
+// tree_size, direction octant), in memory from hipMallocAsync on `stream`
+// that the caller returns with hipFreeAsync on the same stream.
+int order_rays(const float* rays, uint32_t n, float tree_size, uint32_t** order, void* stream,
+               std::string* err);
+// Ray counts from which svo_cast traverses in that order (0: never).
+void set_cast_order_threshold(uint32_t n);
+uint32_t cast_order_threshold();
 int device_render(const DeviceTree& t, const Camera& cam, uint32_t w, uint32_t h,
                   const svo_hits& out, void* stream, std::string* err);
 // get_by_ray_at_lod over a ray array / a viewport's primary rays

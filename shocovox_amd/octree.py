@@ -729,6 +729,12 @@ def set_timing(enabled: bool) -> None:
     N.lib().svo_set_timing(int(enabled))
 
 
+def set_cast_order_threshold(n: int) -> None:
+    """svo_cast orders its rays (origin cell, direction octant) from n rays up
+    (default 2^20; 0 never). Process-wide."""
+    _check(N.lib().svo_cast_set_order_threshold(int(n)))
+
+
 def collect_kernel_ms(cap: int = 4096) -> list:
     buf = (C.c_float * cap)()
     n = N.lib().svo_collect_kernel_ms(buf, cap)

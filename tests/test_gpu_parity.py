@@ -149,6 +149,26 @@ def test_caves_incoherent(gpu_device):
     assert_parity(exp, got, "caves incoherent")
 
 
+def test_cast_in_ray_order_equals_oracle(gpu_device):
+    """svo_cast's traversal order for large ray sets (origin cell, direction
+    octant, by a radix sort on the stream) changes no result: the same rays,
+    with the order forced on (threshold 1) and off, each equal the oracle ray
+    for ray, including rays with NaN, infinite and outside origins."""
+    from shocovox_amd import set_cast_order_threshold
+    tree = W.build_scene(W.CAVES, seed=3)
+    rays = W.incoherent_rays(W.CAVES.size, 120_000, seed=9)
+    rays[:7, :3] = [[np.nan, 1, 1], [np.inf, 5, 5], [-50, 20, 30], [600, 600, 600],
+                    [-np.inf, 0, 0], [1e30, -1e30, 3], [0, 0, 0]]
+    exp = _oracle(tree).cast(rays, threads=16)
+    try:
+        for threshold in (1, 0):
+            set_cast_order_threshold(threshold)
+            _say(f"ordered cast, threshold {threshold}, {len(rays)} rays")
+            assert_parity(exp, _cast(tree, rays, gpu_device), f"order threshold {threshold}")
+    finally:
+        set_cast_order_threshold(1 << 20)
+
+
 def test_axis_aligned_and_degenerate_rays(gpu_device):
     """Rays along axes and planes (inf/NaN scale factors, signed zeros)."""
     tree = W.build_scene(W.PLAIN, seed=1)
