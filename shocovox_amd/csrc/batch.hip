@@ -57,6 +57,9 @@ struct alignas(16) BatchJob {
   uint32_t pad[2];
   Camera cam;
   const float* rays;
+  // cast jobs: the traversal order of the rays (lane k of the job's tiles
+  // takes ray order[k], order.hip), or NULL for the rays' own order
+  const uint32_t* order;
   HitPtrs out;
 };
 
@@ -187,6 +190,7 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
       } else {
         idx = local * 64u + lane;
         valid = idx < job.width;
+        if (valid && job.order) idx = job.order[idx];
         const float* rr = job.rays + 6ull * (valid ? idx : 0u);
         o = V3f{rr[0], rr[1], rr[2]};
         d = V3f{rr[3], rr[4], rr[5]};
@@ -218,6 +222,7 @@ void batch_kernel(const char* __restrict__ table, Counter* __restrict__ counters
           idx = job.tile_major ? again * 64u + lane : y * job.width + x;
         } else {
           idx = again * 64u + lane;
+          if (job.order) idx = job.order[idx];
           const float* rr = job.rays + 6ull * idx;
           o = V3f{rr[0], rr[1], rr[2]};
         }
@@ -279,6 +284,7 @@ struct BatchVariant {
   BatchFn lod_fn;          // the form runs with get_by_ray_at_lod jobs use
   uint32_t groups_per_cu;  // 256-lane workgroups per CU in the persistent grid
   uint32_t tile_w, tile_h; // pixel tile of render jobs
+  bool orders_casts = true;  // its kernel takes a cast job's traversal order
 };
 
 // runs that hold get_by_ray_at_lod jobs use the LOD form of the variant's
@@ -290,8 +296,8 @@ constexpr BatchFn kLod4x16 = batch_kernel<1, false, true, 4, 16>;
 const BatchVariant kBatchVariants[] = {
     {"classic-w1", batch_kernel<1>, kLod8x8, 4, 8, 8},
     {"classic-w5", batch_kernel<5>, kLod8x8, 4, 8, 8},
-    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>, kLod8x8, 4, 8, 8},
-    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>, kLod8x8, 4, 8, 8},
+    {"r1-t4-f16", batch_refill_kernel<1, 4, 16>, kLod8x8, 4, 8, 8, false},
+    {"r1-t8-f16", batch_refill_kernel<1, 8, 16>, kLod8x8, 4, 8, 8, false},
     {"classic-w5-g5", batch_kernel<5>, kLod8x8, 5, 8, 8},
     {"ww-w5", batch_kernel<5, true>, kLod8x8, 4, 8, 8},
     {"ww-w6-g6", batch_kernel<6, true>, kLod8x8, 6, 8, 8},
@@ -590,25 +596,41 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
                             kBatchVariants[b->variant].tile_h, &tiles, &any_lod, err);
     if (st != SVO_OK) return st;
   }
+  // large cast jobs traverse their rays in svo_cast's order (kernels on `s`
+  // ahead of the graph; the orders go back to the pool after it)
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<uint32_t*> orders;
+  if (kBatchVariants[b->variant].orders_casts)
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t threshold = cast_order_threshold();
+      if (jobs[i].kind != 1 || threshold == 0 || jobs[i].width < threshold) continue;
+      uint32_t* ord = nullptr;
+      const int st = order_rays(jobs[i].rays, jobs[i].width, jobs[i].tree->view.size, &ord, s, err);
+      if (st != SVO_OK) {
+        for (uint32_t* p : orders) (void)hipFreeAsync(p, s);
+        return st;
+      }
+      orders.push_back(ord);
+      out[i].order = ord;
+    }
   fill_header(base, n, tiles, trees_offset);
   b->last_tiles = tiles;
   const size_t used = trees_offset + tree_index.size() * sizeof(TreeView);
   if (any_lod && !b->exec_lod) {
     const int st = build_graph(b, kBatchVariants[b->variant].lod_fn, &b->graph_lod,
                                &b->copy_node_lod, &b->exec_lod, err);
-    if (st != SVO_OK) return st;
+    if (st != SVO_OK) {
+      for (uint32_t* p : orders) (void)hipFreeAsync(p, s);
+      return st;
+    }
   }
   hipGraphExec_t exec = any_lod ? b->exec_lod : b->exec;
   hipError_t e = hipGraphExecMemcpyNodeSetParams1D(
       exec, any_lod ? b->copy_node_lod : b->copy_node, b->dev, base, used, hipMemcpyHostToDevice);
-  if (e != hipSuccess) return hip_fail(e, "hipGraphExecMemcpyNodeSetParams1D", err);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  if (b->pending[slot ^ 1u]) {
-    e = hipStreamWaitEvent(s, b->done[slot ^ 1u], 0);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamWaitEvent", err);
-  }
-  e = hipGraphLaunch(exec, s);
-  if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch", err);
+  if (e == hipSuccess && b->pending[slot ^ 1u]) e = hipStreamWaitEvent(s, b->done[slot ^ 1u], 0);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, s);
+  for (uint32_t* p : orders) (void)hipFreeAsync(p, s);
+  if (e != hipSuccess) return hip_fail(e, "batch launch", err);
   e = hipEventRecord(b->done[slot], s);
   if (e != hipSuccess) return hip_fail(e, "hipEventRecord", err);
   b->pending[slot] = true;

@@ -160,11 +160,23 @@ def test_cast_in_ray_order_equals_oracle(gpu_device):
     rays[:7, :3] = [[np.nan, 1, 1], [np.inf, 5, 5], [-50, 20, 30], [600, 600, 600],
                     [-np.inf, 0, 0], [1e30, -1e30, 3], [0, 0, 0]]
     exp = _oracle(tree).cast(rays, threads=16)
+    import torch
+    from shocovox_amd import Batch, alloc_hits
+    d = DeviceTree(tree)
+    t = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(gpu_device)
     try:
         for threshold in (1, 0):
             set_cast_order_threshold(threshold)
             _say(f"ordered cast, threshold {threshold}, {len(rays)} rays")
             assert_parity(exp, _cast(tree, rays, gpu_device), f"order threshold {threshold}")
+            # a batched run's cast job takes the same order
+            _say(f"batched cast job, threshold {threshold}")
+            out = alloc_hits(len(rays), gpu_device)
+            bt = Batch(2)
+            bt.run([Batch.cast_job(d, t, out)])
+            torch.cuda.synchronize()
+            assert_parity(exp, gpu_to_numpy(out), f"batched, order threshold {threshold}")
+            del bt
     finally:
         set_cast_order_threshold(1 << 20)
 
