@@ -303,15 +303,18 @@ typedef struct svo_hits {
 } svo_hits;
 
 /* Octree::get_by_ray for n rays (src/raytracing/raytracing_on_cpu.rs:316-318).
- * rays: device array of n x {origin.xyz, direction.xyz} f32. From
- * svo_cast_order_threshold rays up, the rays are traversed in the order of
+ * rays: device array of n x {origin.xyz, direction.xyz} f32. From the
+ * threshold of svo_cast_set_order_threshold up, the rays are traversed in the order of
  * (origin cell on a 64^3 grid of the tree, Morton order; direction octant),
  * found by a radix sort on `stream` with scratch from the stream-ordered
  * allocator; each ray's outputs are its own in any order. */
 int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hits* out,
              void* stream);
 /* No counterpart in the reference (a tuning knob): the ray count from which
- * svo_cast orders its rays (default 2^20; 0 never orders). Process-wide. */
+ * svo_cast, and svo_batch_run for its cast jobs, order the rays (default
+ * 2^20; 0 never orders; above 2^31 - 1 rays stay unordered). Process-wide.
+ * The sort's scratch comes from the device's default memory pool, which is
+ * set to keep freed memory for reuse. svo_batch_run_static never orders. */
 int svo_cast_set_order_threshold(uint32_t n);
 
 /* Viewport (src/raytracing/bevy/types.rs:55-71): frustum = (glass width,

@@ -110,7 +110,8 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
 // svo_cast's traversal order (order.hip): *order receives n ray indices sorted
 // by (Morton code of the origin on a 64^3 grid of a tree of size
 // tree_size, direction octant), in memory from hipMallocAsync on `stream`
-// that the caller returns with hipFreeAsync on the same stream.
+// that the caller returns with hipFreeAsync on the same stream (*order stays
+// NULL, and the rays in their own order, for n above 2^31 - 1).
 int order_rays(const float* rays, uint32_t n, float tree_size, uint32_t** order, void* stream,
                std::string* err);
 // Ray counts from which svo_cast traverses in that order (0: never).

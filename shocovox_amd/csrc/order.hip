@@ -72,6 +72,7 @@ int hip_fail(hipError_t e, const char* what, std::string* err) {
 int order_rays(const float* rays, uint32_t n, float tree_size, uint32_t** order, void* stream,
                std::string* err) {
   *order = nullptr;
+  if (n > 0x7FFFFFFFu) return SVO_OK;  // the sort counts items in an int: left unordered
   hipStream_t s = static_cast<hipStream_t>(stream);
   {
     // the device's default pool keeps what the sort frees (by default it
