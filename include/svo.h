@@ -314,7 +314,8 @@ int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hit
  * svo_cast, and svo_batch_run for its cast jobs, order the rays (default
  * 2^20; 0 never orders; above 2^31 - 1 rays stay unordered). Process-wide.
  * The sort's scratch comes from the device's default memory pool, which is
- * set to keep freed memory for reuse. svo_batch_run_static never orders. */
+ * set to keep freed memory for reuse; without device memory for the sort the
+ * rays keep their own order. svo_batch_run_static never orders. */
 int svo_cast_set_order_threshold(uint32_t n);
 
 /* Viewport (src/raytracing/bevy/types.rs:55-71): frustum = (glass width,

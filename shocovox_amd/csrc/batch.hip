@@ -606,10 +606,12 @@ int batch_run(BatchContext* b, const BatchJobIn* jobs, uint32_t n, void* stream,
       if (jobs[i].kind != 1 || threshold == 0 || jobs[i].width < threshold) continue;
       uint32_t* ord = nullptr;
       const int st = order_rays(jobs[i].rays, jobs[i].width, jobs[i].tree->view.size, &ord, s, err);
+      if (st == SVO_OUT_OF_DEVICE_MEMORY) continue;  // this job keeps its rays' own order
       if (st != SVO_OK) {
         for (uint32_t* p : orders) (void)hipFreeAsync(p, s);
         return st;
       }
+      if (!ord) continue;
       orders.push_back(ord);
       out[i].order = ord;
     }

@@ -964,8 +964,9 @@ int device_cast(const DeviceTree& t, const float* rays, uint32_t n, const svo_hi
   uint32_t* order = nullptr;
   const uint32_t threshold = cast_order_threshold();
   if (threshold != 0 && n >= threshold) {
+    // without device memory for the sort the rays keep their own order
     const int st = order_rays(rays, n, t.view.size, &order, s, err);
-    if (st != SVO_OK) return st;
+    if (st != SVO_OK && st != SVO_OUT_OF_DEVICE_MEMORY) return st;
   }
   // n < 2^32, so the grid (at most 2^24 workgroups) is computed without
   // wrapping in 64 bits
