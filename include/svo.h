@@ -311,7 +311,7 @@ typedef struct svo_hits {
 int svo_cast(const svo_tree* dtree, const float* rays, uint32_t n, const svo_hits* out,
              void* stream);
 /* No counterpart in the reference (a tuning knob): the ray count from which
- * svo_cast, and svo_batch_run for its cast jobs, order the rays (default
+ * svo_cast, svo_cast_at_lod and svo_batch_run's cast jobs order the rays (default
  * 2^20; 0 never orders; above 2^31 - 1 rays stay unordered). Process-wide.
  * The sort's scratch comes from the device's default memory pool, which is
  * set to keep freed memory for reuse; without device memory for the sort the

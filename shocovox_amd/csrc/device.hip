@@ -42,9 +42,11 @@ __global__ __launch_bounds__(256) void cast_kernel(TreeView t, const float* __re
 // distance when the tree's MIPs are enabled).
 __global__ __launch_bounds__(256) void cast_lod_kernel(TreeView t, const float* __restrict__ rays,
                                                        uint32_t n, float viewing_distance,
+                                                       const uint32_t* __restrict__ order,
                                                        HitPtrs out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = order ? order[k] : k;  // as in cast_kernel
   const float* r = rays + 6ull * i;
   const V3f o{r[0], r[1], r[2]};
   const V3f d{r[3], r[4], r[5]};
@@ -997,10 +999,17 @@ int device_cast_lod(const DeviceTree& t, const float* rays, uint32_t n, float vi
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   LaunchTimer timer(s);
+  uint32_t* order = nullptr;
+  const uint32_t threshold = cast_order_threshold();
+  if (threshold != 0 && n >= threshold) {
+    const int st = order_rays(rays, n, t.view.size, &order, s, err);
+    if (st != SVO_OK && st != SVO_OUT_OF_DEVICE_MEMORY) return st;
+  }
   const uint32_t grid = static_cast<uint32_t>((static_cast<uint64_t>(n) + 255u) / 256u);
   hipLaunchKernelGGL(cast_lod_kernel, dim3(grid), dim3(256), 0, s, t.view, rays, n,
-                     viewing_distance, to_ptrs(out));
+                     viewing_distance, order, to_ptrs(out));
   hipError_t e = hipGetLastError();
+  if (order) (void)hipFreeAsync(order, s);
   if (e != hipSuccess) return hip_fail(e, "cast_lod_kernel launch", err);
   timer.done();
   return SVO_OK;

@@ -690,10 +690,17 @@ def test_lod_traversal_on_device(gpu_device, vd):
         torch.cuda.synchronize()
         o = _oracle(tree)
         assert_parity(o.cast(prim, viewing_distance=vd), got, f"lod render vd={vd} r{rnd}")
-        _say(f"lod vd={vd} round {rnd} cast")
-        got = gpu_to_numpy(d.cast_at_lod(torch.from_numpy(inc).to(gpu_device), vd))
-        torch.cuda.synchronize()
-        assert_parity(o.cast(inc, viewing_distance=vd), got, f"lod cast vd={vd} r{rnd}")
+        exp = o.cast(inc, viewing_distance=vd)
+        from shocovox_amd import set_cast_order_threshold
+        try:
+            # the second round casts in the sorted ray order (forced on)
+            set_cast_order_threshold(1 if rnd else 0)
+            _say(f"lod vd={vd} round {rnd} cast, ray order {'on' if rnd else 'off'}")
+            got = gpu_to_numpy(d.cast_at_lod(torch.from_numpy(inc).to(gpu_device), vd))
+            torch.cuda.synchronize()
+        finally:
+            set_cast_order_threshold(1 << 20)
+        assert_parity(exp, got, f"lod cast vd={vd} r{rnd}")
         xyz = rng.integers(0, 256, size=(500, 3)).astype(np.uint32)
         ents = np.zeros((500, 3), np.uint32)
         ents[:, 0] = 1
