@@ -68,8 +68,12 @@ SVO_HD inline uint32_t occ_word(const TreeView& t, uint32_t brick_off, uint32_t 
   // 0x00020000: 32-bit data format, the word-read descriptor of gfx9 parts
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<uint32_t*>(t.brick_occ), 0, static_cast<int>(t.occ_range), 0x00020000);
-  return __builtin_amdgcn_raw_buffer_load_b32(rsrc, static_cast<int>(brick_off + ((flat >> 5) << 2)),
-                                              0, 0);
+  // (the word index made opaque: the offset is then one shift-add of the
+  // brick's word offset, where the compiler would otherwise form
+  // (flat >> 3) & ~3 and add it separately)
+  uint32_t w = flat >> 5;
+  asm volatile("" : "+v"(w));
+  return __builtin_amdgcn_raw_buffer_load_b32(rsrc, static_cast<int>(brick_off + (w << 2)), 0, 0);
 #else
   return t.brick_occ[(brick_off >> 2) + (in ? (flat >> 5) : 0u)];
 #endif
@@ -118,6 +122,18 @@ SVO_HD inline V3f div3_size(V3f v, float s, float rcp) {
     return div3_by(v, s, rcp);
   }
 }
+// st * 4 / s, the move of a node's bitmap position for a DDA step st (per
+// axis +0.0, +-1 or NaN). With P2, 4 / s is a power of two and both
+// products are exact, so one multiplication gives the same value.
+template <bool P2>
+SVO_HD inline V3f bitmap_move(V3f st, float s, float rcp) {
+  if constexpr (P2) {
+    const float k = rcp * 4.0f;
+    return V3f{st.x * k, st.y * k, st.z * k};
+  } else {
+    return div3_by(st * 4.0f, s, rcp);
+  }
+}
 // a - fmodf(a, b) for a bounds coordinate a >= +0 and b twice a bounds size
 template <bool P2>
 SVO_HD inline float sub_fmod_size(float a, float b, float rcp_b) {
@@ -156,11 +172,13 @@ SVO_HD inline float rclamp(float x, float lo, float hi) {
 // BITMAP_INDEX_LUT[floor(x) as usize][floor(y)][floor(z)] = x + 4y + 16z
 // The reference converts with `as usize`; a saturating u32 conversion gives
 // the same value for 0..3 and a value above 3 (flagged) for everything else,
-// without the 64-bit conversion sequence.
+// without the 64-bit conversion sequence. The conversion truncates, which
+// saturates to the same value as floor would: equal for x >= 0, and 0 for
+// negative x and NaN either way, so the floor is left out.
 SVO_HD inline uint32_t bitmap_index(V3f bp, uint32_t* flags) {
-  uint32_t x = f2u32(floorf(bp.x));
-  uint32_t y = f2u32(floorf(bp.y));
-  uint32_t z = f2u32(floorf(bp.z));
+  uint32_t x = f2u32(bp.x);
+  uint32_t y = f2u32(bp.y);
+  uint32_t z = f2u32(bp.z);
   if (x > 3 || y > 3 || z > 3) {
     *flags |= FLAG_BITMAP_RANGE;
     x = x > 3 ? 3 : x;
@@ -193,14 +211,14 @@ SVO_HD inline CastResult miss_result(uint32_t flags) {
 }
 
 // bitmap_index(rclamp(bp, 1e-5, 4 - 1e-5)) without forming the clamped
-// position: per axis, floor then a saturating conversion (NaN -> 0) and a
+// position: per axis, a saturating conversion (truncation, NaN -> 0) and a
 // clamp of the integer to 0..3 give the same cell as clamping first (below
-// 1e-5 and above 4 - 1e-5 the floor is <= 0 and >= 3; NaN passes the float
+// 1e-5 and above 4 - 1e-5 the integer is <= 0 and >= 3; NaN passes the float
 // clamp and converts to 0 either way), and no range test is needed.
+// Truncation and floor differ only below 0, where both clamp to 0.
 SVO_HD inline uint32_t clamp03(int32_t v) { return static_cast<uint32_t>(v < 0 ? 0 : (v > 3 ? 3 : v)); }
 SVO_HD inline uint32_t bitmap_cell(V3f bp) {
-  return clamp03(f2i32(floorf(bp.x))) + 4u * clamp03(f2i32(floorf(bp.y))) +
-         16u * clamp03(f2i32(floorf(bp.z)));
+  return clamp03(f2i32(bp.x)) + 4u * clamp03(f2i32(bp.y)) + 16u * clamp03(f2i32(bp.z));
 }
 
 // probe_brick (raytracing_on_cpu.rs:256-312) with traverse_brick (:156-252)
@@ -219,16 +237,23 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
   const int dim = static_cast<int>(t.dim);
   const float fdim = static_cast<float>(t.dim);
   const float rcp_size = size_rcp<P2>(bb.size);
-  const V3f rel = (*p - bb.min) * fdim;
-  const V3f pib = div3_size<P2>(rel, bb.size, rcp_size);
-  auto clampi = [dim](int v) { return v < 0 ? 0 : (v > dim - 1 ? dim - 1 : v); };
+  V3f pib;
+  if constexpr (P2) {
+    // (p - min) * dim / size: dim and 1 / size are powers of two, so scaling
+    // by their product rounds the same real number once, as the two steps do
+    const float k = fdim * rcp_size;
+    pib = (*p - bb.min) * k;
+  } else {
+    pib = div3_size<P2>((*p - bb.min) * fdim, bb.size, rcp_size);
+  }
+  auto clampi = [dim](int v) {
+    const int a = v > 0 ? v : 0;
+    return a < dim - 1 ? a : dim - 1;
+  };
   int ix = clampi(f2i32(pib.x));
   int iy = clampi(f2i32(pib.y));
   int iz = clampi(f2i32(pib.z));
   const float unit = div_size<P2>(bb.size, fdim, t.inv_dim);
-  V3f cmin = bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy), static_cast<float>(iz)} *
-                          unit;
-  const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
   // A step moves the cell by signum(direction) on the axes that stepped and
   // leaves the others (their step component is +0.0, and cell corners are
   // never -0.0): the per-axis moves are fixed for the whole probe.
@@ -262,10 +287,19 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
     uint32_t ax = static_cast<uint32_t>(ix) ^ nx;
     uint32_t ay = static_cast<uint32_t>(iy) ^ ny;
     uint32_t az = static_cast<uint32_t>(iz) ^ nz;
-    const uint32_t occ_off = word * t.occ_words * 4u;
-    // cmin holds the cell's far corner (the corner the direction points to)
-    cmin = V3f{cmin.x + unit * rmax(sig.x, 0.0f), cmin.y + unit * rmax(sig.y, 0.0f),
-               cmin.z + unit * rmax(sig.z, 0.0f)};
+    // occupancy words per brick: dim^3 / 32 rounded up, a power of two
+    const uint32_t occ_off = (word << static_cast<uint32_t>(__builtin_ctz(t.occ_words))) * 4u;
+    // cmin holds the cell's far corner (the corner the direction points to):
+    // min + (i + 1) unit on an axis whose signum is +1, min + i unit on the
+    // others. Every term is a multiple of unit (a power of two) no larger
+    // than the tree size (at most 2^24), so each is exact in f32: the fused
+    // form gives the same value as the reference's min + i unit, then + unit.
+    V3f cmin{fmaf(static_cast<float>(static_cast<uint32_t>(ix) + (sig.x < 0.0f ? 0u : 1u)), unit,
+                  bb.min.x),
+             fmaf(static_cast<float>(static_cast<uint32_t>(iy) + (sig.y < 0.0f ? 0u : 1u)), unit,
+                  bb.min.y),
+             fmaf(static_cast<float>(static_cast<uint32_t>(iz) + (sig.z < 0.0f ? 0u : 1u)), unit,
+                  bb.min.z)};
     for (;;) {
       in = (ax | ay | az) < t.dim;
       const uint32_t flat = (ax | (ay << shift) | (az << (2u * shift))) ^ mirror;
@@ -290,6 +324,10 @@ SVO_HD __attribute__((always_inline)) inline bool probe_brick(const TreeView& t,
     iy = static_cast<int>(ay ^ ny);
     iz = static_cast<int>(az ^ nz);
   } else {
+    V3f cmin = bb.min + V3f{static_cast<float>(ix), static_cast<float>(iy),
+                            static_cast<float>(iz)} *
+                            unit;
+    const uint32_t* occ = t.brick_occ + static_cast<uint64_t>(word) * t.occ_words;
     const int mx = step_int(sig.x), my = step_int(sig.y), mz = step_int(sig.z);
     for (int left = 4 * dim + 16;;) {
       in = ix >= 0 && ix < dim && iy >= 0 && iy < dim && iz >= 0 && iz < dim;
@@ -433,7 +471,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
 #if defined(__HIP_DEVICE_COMPILE__)
     asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
 #endif
-    const V3f sig{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
+    V3f sig{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
     if (++steps > t.max_steps) {
       r.flags |= FLAG_ITERATION_GUARD;
       break;
@@ -464,6 +502,13 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
                           &p, &r))
         goto hit_found;
     }
+    // signum(d) formed again past the probe, so that the values formed
+    // before it are dead inside the probe's DDA loop (whose registers then
+    // hold the per-axis cell moves rather than recomputing them each step)
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(d.x), "+v"(d.y), "+v"(d.z));
+#endif
+    sig = V3f{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
     const float rcp_size = size_rcp<P2>(bounds.size);
     const V3f rel = (p - bounds.min) * 4.0f;
     // the position in the node's 4x4x4 bitmap; the clamp to [lo, hi]
@@ -535,8 +580,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
         bp = V3f{rclamp(bp.x, lo, hi), rclamp(bp.y, lo, hi), rclamp(bp.z, lo, hi)};
         while (next_target != OOB_OCTANT) {
           tb = child_bounds_for(bounds, next_target);
-          const V3f sd = st * 4.0f;
-          bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
+          bp = bp + bitmap_move<P2>(st, bounds.size, rcp_size);
           flat = bitmap_index(bp, &r.flags);
           // stop at a reachable occupied child or a non-empty leaf brick; the
           // tests run cheapest first and the child word is read last
@@ -717,8 +761,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
           target = step_octant(target, st);
           if (target == OOB_OCTANT) break;
           tb = child_bounds_for(bounds, target);
-          const V3f sd = st * 4.0f;
-          bp = bp + div3_size<P2>(sd, bounds.size, rcp_size);
+          bp = bp + bitmap_move<P2>(st, bounds.size, rcp_size);
           flat = bitmap_index(bp, &r.flags);
           if (kind == KIND_LEAF && brick_form(meta, target) != FORM_EMPTY) break;
           if (kind == KIND_INTERNAL && (occ & octant_bitmap_mask(target)) != 0ull &&
