@@ -170,9 +170,14 @@ struct Cube {
   float size;
 };
 
+// (min + offset * half per axis: the offset is 0 or 1, so the product is
+// exact and the fused form rounds the same sum once)
 SVO_HD inline Cube child_bounds_for(Cube c, uint32_t octant) {
   const float child_size = c.size / 2.0f;
-  return Cube{c.min + octant_offset(octant) * child_size, child_size};
+  const V3f o = octant_offset(octant);
+  return Cube{V3f{fmaf(o.x, child_size, c.min.x), fmaf(o.y, child_size, c.min.y),
+                  fmaf(o.z, child_size, c.min.z)},
+              child_size};
 }
 
 // Cube::intersect_ray (raytracing/mod.rs:32-61).

@@ -134,6 +134,20 @@ SVO_HD inline V3f bitmap_move(V3f st, float s, float rcp) {
     return div3_by(st * 4.0f, s, rcp);
   }
 }
+// The POP's octant of the node within its parent (raytracing_on_cpu.rs:
+// 450-461): hash_region(center - parent min, size). With P2 every bounds
+// coordinate is a multiple of its size below 2^24 and min - parent min is
+// exactly 0 or the size per axis, so the test center - parent min >= size
+// is min != parent min.
+template <bool P2>
+SVO_HD inline uint32_t pop_octant(Cube b, V3f pmin) {
+  if constexpr (P2) {
+    return static_cast<uint32_t>(b.min.x != pmin.x) + static_cast<uint32_t>(b.min.z != pmin.z) * 2u +
+           static_cast<uint32_t>(b.min.y != pmin.y) * 4u;
+  } else {
+    return hash_region(b.min + v3u(b.size / 2.0f) - pmin, b.size);
+  }
+}
 // a - fmodf(a, b) for a bounds coordinate a >= +0 and b twice a bounds size
 template <bool P2>
 SVO_HD inline float sub_fmod_size(float a, float b, float rcp_b) {
@@ -551,13 +565,12 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
           return crawl_out_of_root(t, p, d, steps, r.flags);
         if (stk.count > 1u) {
           // the parent's bounds and the sibling the step leads to
-          const V3f center = bounds.min + v3u(bounds.size / 2.0f);
           const float s2 = bounds.size * 2.0f;
           const float rcp_s2 = size_rcp<P2>(s2);
           const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
                          sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
                          sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
-          const uint32_t from = hash_region(center - pmin, bounds.size);
+          const uint32_t from = pop_octant<P2>(bounds, pmin);
           next_target = step_octant(from, st);
           next_bounds = Cube{pmin, s2};
         } else {
@@ -726,13 +739,12 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
         }
         stk.pop();
         if (stk.count != 0u) {
-          const V3f center = bounds.min + v3u(bounds.size / 2.0f);
           const float s2 = bounds.size * 2.0f;
           const float rcp_s2 = size_rcp<P2>(s2);
           const V3f pmin{sub_fmod_size<P2>(bounds.min.x, s2, rcp_s2),
                          sub_fmod_size<P2>(bounds.min.y, s2, rcp_s2),
                          sub_fmod_size<P2>(bounds.min.z, s2, rcp_s2)};
-          const uint32_t from = hash_region(center - pmin, bounds.size);
+          const uint32_t from = pop_octant<P2>(bounds, pmin);
           const V3f st = dda_step(d, sig, &p, bounds, scale);
           target = step_octant(from, st);
           bounds.size *= 2.0f;
