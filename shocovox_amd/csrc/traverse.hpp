@@ -122,6 +122,19 @@ SVO_HD inline V3f div3_size(V3f v, float s, float rcp) {
     return div3_by(v, s, rcp);
   }
 }
+// (p - min) * 4 / size, the position of p in the 4x4x4 bitmap of the node
+// with bounds b (raytracing_on_cpu.rs:425-427). With P2, 4 and 1 / size are
+// powers of two, so one scaling by their product rounds the same real
+// number once, as the two steps do.
+template <bool P2>
+SVO_HD inline V3f bitmap_pos(V3f p, Cube b, float rcp) {
+  if constexpr (P2) {
+    const float k = rcp * 4.0f;
+    return V3f{(p.x - b.min.x) * k, (p.y - b.min.y) * k, (p.z - b.min.z) * k};
+  } else {
+    return div3_by((p - b.min) * 4.0f, b.size, rcp);
+  }
+}
 // st * 4 / s, the move of a node's bitmap position for a DDA step st (per
 // axis +0.0, +-1 or NaN). With P2, 4 / s is a power of two and both
 // products are exact, so one multiplication gives the same value.
@@ -524,11 +537,10 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_impl(
 #endif
     sig = V3f{sign_one(d.x), sign_one(d.y), sign_one(d.z)};
     const float rcp_size = size_rcp<P2>(bounds.size);
-    const V3f rel = (p - bounds.min) * 4.0f;
     // the position in the node's 4x4x4 bitmap; the clamp to [lo, hi]
     // (:425-436) is applied to the float position only where ADVANCE
     // continues from it, the cell index comes from bitmap_cell
-    V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
+    V3f bp = bitmap_pos<P2>(p, bounds, rcp_size);
     uint32_t flat = bitmap_cell(bp);
     const bool pop = backtrack || target == OOB_OCTANT || occ == 0ull ||
                      (occ & lut[flat * 8u + dir_idx]) == 0ull;
@@ -723,8 +735,7 @@ SVO_HD __attribute__((always_inline)) inline CastResult cast_ray_ww(
       probed = false;
       const bool backtrack = kind == KIND_UNIFORM_LEAF;
       const float rcp_size = size_rcp<P2>(bounds.size);
-      const V3f rel = (p - bounds.min) * 4.0f;
-      V3f bp = div3_size<P2>(rel, bounds.size, rcp_size);
+      V3f bp = bitmap_pos<P2>(p, bounds, rcp_size);
       uint32_t flat = bitmap_cell(bp);  // cell of the clamped position
       if (backtrack || target == OOB_OCTANT || occ == 0ull ||
           (occ & lut[flat * 8u + dir_idx]) == 0ull) {
